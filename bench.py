@@ -1,0 +1,208 @@
+"""Benchmark: multislice mixed-probe ptychography forward + adjoint throughput on one MI355X.
+
+Workload (8 slices of 384x384, 2 A thick; 128x128 patterns; 4 probe modes; 64x64 raster scan at a
+4 px step with +-0.5 px jitter = 4096 patterns; 300 kV; Poisson data at 1e5 e-/pattern; complex64;
+batch 256).  One step = one pass over all 4096 patterns in 16 batches, each batch running what the
+reference's reconstruction loop runs per batch (quantem ptychography.py:213-243): fused forward +
+loss, hand-written adjoint (object, probe and scan-position gradients), the object-gradient
+rescale and an Adam step.  Inputs are resident in HBM before the timed region.
+
+Prints one JSON line.  With --gpus N > 1 it exits: this build measures one GPU.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+import torch
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+from quantem_amd import _native  # noqa: E402
+from quantem_amd import ptychography as qp  # noqa: E402
+from quantem_amd import synth  # noqa: E402
+
+METRIC = "patterns/s fwd+adjoint, % HBM roofline; time-to-target-NMSE warm-start vs flat"
+FP32_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def fft_flops(n: int) -> float:
+    """Textbook complex 2-D FFT flop count 5 N^2 log2(N^2)."""
+    return 5.0 * n * n * np.log2(n * n)
+
+
+def make_inputs(spec: synth.ProblemSpec, dev: torch.device, seed: int = 0) -> dict:
+    prob = synth.make_problem(spec, seed=seed)
+    obj_true = torch.from_numpy(prob["obj"]).to(dev)
+    probe = torch.from_numpy(prob["probe"]).to(dev)
+    pos = torch.from_numpy(prob["positions"]).to(dev)
+    props = torch.from_numpy(prob["propagators"]).to(dev)
+    P = pos.shape[0]
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed + 1)
+    amps = torch.empty((P, spec.roi, spec.roi), dtype=torch.float32, device=dev)
+    total = 0.0
+    for i in range(0, P, spec.batch):
+        I = qp.forward_intensities(obj_true, probe, pos[i:i + spec.batch], props)
+        counts = torch.poisson(I.clamp_min(0), generator=gen)
+        total += float(counts.sum().item())
+        amps[i:i + spec.batch] = torch.sqrt(counts)
+    return {"obj_true": obj_true, "probe": probe, "positions": pos, "propagators": props, "targets": amps,
+            "mean_intensity": total / P, "num_gpts": P}
+
+
+def batches(P: int, B: int, seed: int) -> list:
+    perm = np.random.default_rng(seed).permutation(P)
+    return [torch.from_numpy(perm[i:i + B].copy()) for i in range(0, P, B)]
+
+
+def gpu_pass(state: dict, data: dict, idx_list: list, scale: float) -> None:
+    obj, probe, opt = state["obj"], state["probe"], state["opt"]
+    for idx in idx_list:
+        opt.zero_grad(set_to_none=False)
+        loss = qp.multislice_loss(obj, probe, data["positions"][idx], data["propagators"], data["targets"][idx],
+                                  None, None, num_gpts=data["num_gpts"], mean_intensity=data["mean_intensity"])
+        loss.backward()
+        obj.grad.mul_(scale)
+        opt.step()
+
+
+def cpu_baseline(data: dict, spec: synth.ProblemSpec, seconds: float, sample_batch: int) -> dict:
+    """The reference op order on the host cores (complex64 torch, loss.backward()), bounded sample."""
+    from oracle import torch_ref
+
+    threads = os.cpu_count() or 1
+    torch.set_num_threads(threads)
+    idx = torch.arange(sample_batch)
+    obj = torch.ones_like(data["obj_true"]).cpu().requires_grad_(True)
+    probe = data["probe"].cpu().clone().requires_grad_(True)
+    pos = data["positions"][idx].cpu()
+    props = data["propagators"].cpu()
+    tgt = data["targets"][idx.to(data["targets"].device)].cpu()
+
+    def one():
+        obj.grad = None
+        probe.grad = None
+        loss, _ = torch_ref.batch_loss(obj, probe, pos, props, tgt, None, data["num_gpts"], data["mean_intensity"])
+        loss.backward()
+
+    one()  # warm-up
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        one()
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    return {"value": n * sample_batch / el, "unit": "patterns/s", "cores": threads, "kind": "port",
+            "sample": f"{n} batches x {sample_batch} patterns of the same workload (torch {torch.__version__} "
+                      f"complex64 CPU, reference op order, loss.backward()), {el:.1f} s, "
+                      f"{torch.get_num_threads()} threads, {platform.processor() or platform.machine()}"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-batch", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
+              file=sys.stderr, flush=True)
+        return 2
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        return 1
+    dev = torch.device("cuda:0")
+    _native.load()
+    spec = synth.HEADLINE
+    log(f"generating workload {spec.name}")
+    data = make_inputs(spec, dev, args.seed)
+    S, (H, W), N, M, B = spec.num_slices, spec.obj_shape, spec.roi, spec.num_modes, spec.batch
+    P = data["num_gpts"]
+    obj = torch.nn.Parameter(torch.ones((S, H, W), dtype=torch.complex64, device=dev))
+    probe = torch.nn.Parameter(data["probe"].clone())
+    opt = torch.optim.Adam([{"params": [obj], "lr": 5e-3}, {"params": [probe], "lr": 1e-3}])
+    state = {"obj": obj, "probe": probe, "opt": opt}
+    upsample = ((np.asarray(spec.scan) - 1) * spec.scan_step / (np.asarray(spec.scan) - 1)).mean()
+    scale = float(upsample**2 / 2)
+    idx_list = [i.to(dev) for i in batches(P, B, args.seed)]
+
+    log(f"warmup {args.warmup} steps")
+    for _ in range(args.warmup):
+        gpu_pass(state, data, idx_list, scale)
+    torch.cuda.synchronize()
+    log(f"timing {args.steps} steps")
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with _native.KernelTimer(max_launches=len(idx_list) * args.steps + 8) as kt:
+        torch.cuda.synchronize()
+        ev0.record()
+        for _ in range(args.steps):
+            gpu_pass(state, data, idx_list, scale)
+        ev1.record()
+        torch.cuda.synchronize()
+    total_ms = ev0.elapsed_time(ev1)
+    ms_per_step = total_ms / args.steps
+    value = P * args.steps / (total_ms / 1e3)
+    log(f"{ms_per_step:.3f} ms/step, {value:.0f} patterns/s; fwd kernel {kt.fwd_ms / max(kt.n_fwd, 1):.3f} ms x "
+        f"{kt.n_fwd}, adjoint kernel {kt.adj_ms / max(kt.n_adj, 1):.3f} ms x {kt.n_adj}")
+
+    # roofline of the dominant kernel (FFT flops; see DESIGN.md for the unit counts)
+    flops_per_launch = B * M * 2 * S * fft_flops(N)
+    t_fwd = kt.fwd_ms / max(kt.n_fwd, 1)
+    t_adj = kt.adj_ms / max(kt.n_adj, 1)
+    dom, t_dom = ("multislice adjoint kernel", t_adj) if t_adj >= t_fwd else ("multislice forward kernel", t_fwd)
+    achieved = flops_per_launch / (t_dom / 1e3) / 1e12
+    roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+            "kernel": dom, "kernel_ms": round(t_dom, 4),
+            "fwd_kernel_ms": round(t_fwd, 4), "adj_kernel_ms": round(t_adj, 4),
+            "note": "FP32 vector (FFT-flop) bound, no MFMA used; 157.3 TFLOP/s is the MI355X FP32 peak; "
+                    "algorithmic flops = B*M*2S*5N^2*log2(N^2) per launch"}
+    cpu = None
+    if not args.no_cpu:
+        log(f"cpu baseline ~{args.cpu_seconds:.0f} s")
+        cpu = cpu_baseline(data, spec, args.cpu_seconds, args.cpu_batch)
+        log(f"cpu {cpu['value']:.1f} patterns/s")
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "patterns/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "complex64",
+        "data": "synthetic",
+        "config": {"workload": spec.name, "slices": S, "object": [H, W], "roi": N, "modes": M,
+                   "scan": list(spec.scan), "patterns": P, "batch": B, "parallelism": "single-gpu"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

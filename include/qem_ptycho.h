@@ -1,0 +1,130 @@
+/*
+ * qem_ptycho.h — C ABI of the MI355X (gfx950) multislice mixed-probe ptychography kernels.
+ *
+ * The library replaces, for one mini-batch of scan positions, the reference's per-batch
+ * forward/loss call and its torch-autograd gradient:
+ *
+ *   quantem/diffractive_imaging/ptychography.py:220-229     (dset.forward -> probe/obj forward ->
+ *                                                            forward_operator -> detector -> error_estimate)
+ *   quantem/diffractive_imaging/ptychography.py:327-329     (loss.backward() through all of the above)
+ *
+ * Conventions (all tensors row-major, C-contiguous, device memory, little endian):
+ *   complex64 = interleaved {float re, float im}.
+ *   obj          (S, Ho, Wo)  complex64  transmission of each slice (after the caller's hard constraints)
+ *   probe        (M, N, N)    complex64  probe modes, corner-centred real space
+ *   positions    (B, 2)       float32    scan positions in object pixels (row, col); the integer origin is
+ *                                        round-half-even(pos) and the sub-pixel shift is pos - origin
+ *   propagators  (S-1, N, N)  complex64  Fresnel propagators in unshifted fftfreq order
+ *   targets      (B, N, N)    float32    measured amplitudes (or intensities), fftshifted (DC at N/2, N/2)
+ *   mask         (N, N)       float32    detector mask, fftshifted
+ *   descan       (B, 2)       float32    optional descan shifts in pixels (NULL = none)
+ *
+ * Every call is asynchronous on `stream` (a hipStream_t passed as void*), allocates nothing,
+ * retains no pointer past its return, and gives results that do not depend on launch order
+ * (all reductions run in a fixed order).  Return value 0 = success, otherwise a QEM_ERR_* code;
+ * qem_last_error() returns a thread-local message describing the last failure.
+ */
+#ifndef QEM_PTYCHO_H
+#define QEM_PTYCHO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  QEM_OK = 0,
+  QEM_ERR_ARG = 1,        /* invalid descriptor field or size */
+  QEM_ERR_NULL = 2,       /* required pointer is NULL */
+  QEM_ERR_ALIGN = 3,      /* pointer not aligned to 8 bytes */
+  QEM_ERR_WORKSPACE = 4,  /* workspace smaller than qem_ptycho_workspace_bytes() */
+  QEM_ERR_LAUNCH = 5,     /* HIP launch failure */
+  QEM_ERR_UNSUPPORTED = 6 /* valid request this build does not implement (e.g. roi size) */
+};
+
+/* Loss types of PtychographyBase.error_estimate (ptychography_base.py:908-932). */
+enum {
+  QEM_LOSS_L2_AMPLITUDE = 0,
+  QEM_LOSS_L1_AMPLITUDE = 1,
+  QEM_LOSS_L2_INTENSITY = 2,
+  QEM_LOSS_L1_INTENSITY = 3,
+  QEM_LOSS_POISSON = 4
+};
+
+/* Descriptor flags. */
+enum {
+  QEM_FLAG_STORE_WAVES = 1u << 0, /* keep the per-slice incident waves for qem_ptycho_adjoint */
+  QEM_FLAG_GRAD_POSITIONS = 1u << 1, /* adjoint also writes d loss / d positions */
+  QEM_FLAG_GRAD_DESCAN = 1u << 2     /* adjoint also writes d loss / d descan */
+};
+
+typedef struct qem_ptycho_desc {
+  int32_t num_slices;      /* S  >= 1 */
+  int32_t num_modes;       /* M  >= 1 */
+  int32_t batch;           /* B  >= 1, patterns in this call */
+  int32_t roi;             /* N  in {32, 64, 128} */
+  int32_t obj_rows;        /* Ho >= N */
+  int32_t obj_cols;        /* Wo >= N */
+  int32_t num_gpts;        /* P  total scan positions of the dataset (loss normalisation) */
+  int32_t loss_type;       /* QEM_LOSS_* */
+  float mean_intensity;    /* mean diffraction intensity of the dataset (> 0) */
+  uint32_t flags;          /* QEM_FLAG_* */
+} qem_ptycho_desc;
+
+/* Bytes of device workspace a forward_loss/adjoint pair needs for this descriptor.
+ * No reference counterpart: the reference allocates its temporaries through torch. */
+int qem_ptycho_workspace_bytes(const qem_ptycho_desc* desc, size_t* bytes);
+
+/* Forward model + loss for one batch.
+ * Replaces ProbePixelated.forward (probe_models.py:718-720), ObjectPixelated.forward
+ * (object_models.py:678-680, gather at 252-265), PtychographyBase.forward_operator
+ * (ptychography_base.py:892-906, overlap_projection 934-947, _propagate_array 970-989),
+ * DetectorPixelated.forward (detector_models.py:22-30) and PtychographyBase.error_estimate
+ * (ptychography_base.py:908-932).
+ *   loss        device float[1]  the batch loss
+ *   intensities device float (B, N, N) fftshifted predicted intensities, or NULL
+ * With QEM_FLAG_STORE_WAVES the workspace keeps what qem_ptycho_adjoint needs. */
+int qem_ptycho_forward_loss(const qem_ptycho_desc* desc, const void* obj, const void* probe,
+                            const float* positions, const void* propagators, const float* targets,
+                            const float* mask, const float* descan, float* loss, float* intensities,
+                            void* workspace, size_t workspace_bytes, void* stream);
+
+/* Gradient of the batch loss (torch convention: grad = dL/dRe + i dL/dIm), scaled by *dloss
+ * (device float, the upstream gradient).  Replaces torch autograd of the chain above
+ * (invoked at ptychography.py:328).  Must follow qem_ptycho_forward_loss with the same
+ * descriptor (QEM_FLAG_STORE_WAVES set), inputs and workspace.
+ *   grad_obj       (S, Ho, Wo) complex64, fully written (zero where no patch lands)
+ *   grad_probe     (M, N, N)   complex64
+ *   grad_positions (B, 2) float32 or NULL (needs QEM_FLAG_GRAD_POSITIONS)
+ *   grad_descan    (B, 2) float32 or NULL (needs QEM_FLAG_GRAD_DESCAN and descan != NULL) */
+int qem_ptycho_adjoint(const qem_ptycho_desc* desc, const void* obj, const void* probe,
+                       const float* positions, const void* propagators, const float* descan,
+                       const float* dloss, void* grad_obj, void* grad_probe, float* grad_positions,
+                       float* grad_descan, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Batched 2-D complex64 FFT of `count` N x N arrays with torch.fft.fft2 / ifft2 default
+ * normalisation (forward unscaled, inverse 1/N^2).  Replaces af.fft2 / af.ifft2
+ * (core/utils/array_funcs.py:263-285) for the square power-of-two sizes above. */
+int qem_fft2_c64(int32_t n, int32_t count, int32_t inverse, const void* in, void* out, void* stream);
+
+/* Measurement hook (no reference counterpart; the reference has no timing code): while active,
+ * every forward_loss / adjoint call records a HIP event pair on its stream immediately around its
+ * dominant kernel (the fused multislice forward, resp. adjoint), for at most max_launches calls
+ * each.  qem_profile_end waits for the recorded events and returns the summed kernel times in
+ * milliseconds and the number of timed launches.  Not thread-safe; one session at a time. */
+int qem_profile_begin(int32_t max_launches);
+int qem_profile_end(float* fwd_ms, int32_t* n_fwd, float* adj_ms, int32_t* n_adj);
+
+/* Thread-local description of the last error (empty string if none). */
+const char* qem_last_error(void);
+
+/* ABI version (major * 100 + minor). */
+int32_t qem_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QEM_PTYCHO_H */

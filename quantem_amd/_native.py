@@ -1,0 +1,147 @@
+"""ctypes binding of the C-ABI library ``libqem_ptycho.so`` (declared in ``include/qem_ptycho.h``).
+
+The library must have been built (``quantem_amd.build.build()``); there is no fallback: every
+compute entry point of this package runs the HIP kernels or raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+from .build import LIB_PATH
+
+QEM_OK = 0
+QEM_ERR_ARG = 1
+QEM_ERR_NULL = 2
+QEM_ERR_ALIGN = 3
+QEM_ERR_WORKSPACE = 4
+QEM_ERR_LAUNCH = 5
+QEM_ERR_UNSUPPORTED = 6
+
+QEM_FLAG_STORE_WAVES = 1 << 0
+QEM_FLAG_GRAD_POSITIONS = 1 << 1
+QEM_FLAG_GRAD_DESCAN = 1 << 2
+
+LOSS_TYPES = {
+    "l2_amplitude": 0,
+    "l1_amplitude": 1,
+    "l2_intensity": 2,
+    "l1_intensity": 3,
+    "poisson": 4,
+}
+
+SUPPORTED_ROI = (32, 64, 128)
+
+EXPORTED_SYMBOLS = (
+    "qem_ptycho_workspace_bytes",
+    "qem_ptycho_forward_loss",
+    "qem_ptycho_adjoint",
+    "qem_fft2_c64",
+    "qem_last_error",
+    "qem_abi_version",
+    "qem_profile_begin",
+    "qem_profile_end",
+)
+
+
+class QemDesc(ctypes.Structure):
+    _fields_ = [
+        ("num_slices", ctypes.c_int32),
+        ("num_modes", ctypes.c_int32),
+        ("batch", ctypes.c_int32),
+        ("roi", ctypes.c_int32),
+        ("obj_rows", ctypes.c_int32),
+        ("obj_cols", ctypes.c_int32),
+        ("num_gpts", ctypes.c_int32),
+        ("loss_type", ctypes.c_int32),
+        ("mean_intensity", ctypes.c_float),
+        ("flags", ctypes.c_uint32),
+    ]
+
+
+_lib = None
+_lock = threading.Lock()
+
+_vp = ctypes.c_void_p
+_fp = ctypes.c_void_p  # float* passed as raw device address
+
+
+def _declare(lib: ctypes.CDLL) -> None:
+    dp = ctypes.POINTER(QemDesc)
+    lib.qem_ptycho_workspace_bytes.argtypes = [dp, ctypes.POINTER(ctypes.c_size_t)]
+    lib.qem_ptycho_workspace_bytes.restype = ctypes.c_int
+    lib.qem_ptycho_forward_loss.argtypes = [dp] + [_vp] * 10 + [ctypes.c_size_t, _vp]
+    lib.qem_ptycho_forward_loss.restype = ctypes.c_int
+    lib.qem_ptycho_adjoint.argtypes = [dp] + [_vp] * 11 + [ctypes.c_size_t, _vp]
+    lib.qem_ptycho_adjoint.restype = ctypes.c_int
+    lib.qem_fft2_c64.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp, _vp, _vp]
+    lib.qem_fft2_c64.restype = ctypes.c_int
+    lib.qem_last_error.argtypes = []
+    lib.qem_last_error.restype = ctypes.c_char_p
+    lib.qem_profile_begin.argtypes = [ctypes.c_int32]
+    lib.qem_profile_begin.restype = ctypes.c_int
+    lib.qem_profile_end.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int32),
+                                    ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int32)]
+    lib.qem_profile_end.restype = ctypes.c_int
+    lib.qem_abi_version.argtypes = []
+    lib.qem_abi_version.restype = ctypes.c_int32
+
+
+def lib_path() -> str:
+    return os.environ.get("QEM_PTYCHO_LIB", LIB_PATH)
+
+
+def load() -> ctypes.CDLL:
+    """Loads the HIP library once; raises if it has not been built."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            path = lib_path()
+            if not os.path.exists(path):
+                raise RuntimeError(
+                    f"HIP library {path} is missing: build it with `python -m quantem_amd.build` "
+                    "(this package has no CPU fallback)"
+                )
+            lib = ctypes.CDLL(path)
+            _declare(lib)
+            _lib = lib
+        return _lib
+
+
+def last_error() -> str:
+    msg = load().qem_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(rc: int, what: str) -> None:
+    """Raises the exception class the reference raises for the same kind of failure."""
+    if rc == QEM_OK:
+        return
+    msg = f"{what}: {last_error()} (code {rc})"
+    if rc in (QEM_ERR_ARG, QEM_ERR_NULL, QEM_ERR_ALIGN, QEM_ERR_WORKSPACE, QEM_ERR_UNSUPPORTED):
+        raise ValueError(msg)
+    raise RuntimeError(msg)
+
+
+class KernelTimer:
+    """Context manager timing the fused forward / adjoint kernels with HIP events (library hook)."""
+
+    def __init__(self, max_launches: int = 4096):
+        self.max_launches = max_launches
+        self.fwd_ms = self.adj_ms = 0.0
+        self.n_fwd = self.n_adj = 0
+
+    def __enter__(self):
+        check(load().qem_profile_begin(self.max_launches), "qem_profile_begin")
+        return self
+
+    def __exit__(self, *exc):
+        f, a = ctypes.c_float(0), ctypes.c_float(0)
+        nf, na = ctypes.c_int32(0), ctypes.c_int32(0)
+        check(load().qem_profile_end(ctypes.byref(f), ctypes.byref(nf), ctypes.byref(a), ctypes.byref(na)),
+              "qem_profile_end")
+        self.fwd_ms, self.adj_ms = float(f.value), float(a.value)
+        self.n_fwd, self.n_adj = int(nf.value), int(na.value)
+        return False

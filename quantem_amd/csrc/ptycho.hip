@@ -1,0 +1,841 @@
+// ptycho.hip — fused multislice mixed-probe ptychography forward / loss / adjoint for gfx950.
+//
+// One workgroup owns one (pattern b, probe mode m) wave for the whole multislice chain; the wave
+// stays in registers (layout P1 / P3 of fft2.h) and only the FFT exchanges touch LDS.  HBM sees
+// the object patches (L2-resident across modes), the stored per-slice incident waves (for the
+// adjoint), the far-field wave and the per-pattern targets.
+//
+// Reference behaviour followed (quantem, src/quantem/diffractive_imaging/):
+//   patch origin / fractional shift   dataset_models.py:164-167, 514-539, 1482-1499
+//   object patch gather (periodic)    object_models.py:252-265
+//   sub-pixel probe shift             ptycho_utils.py:128-139, 156-176; probe_models.py:718-720
+//   multislice transmit + propagate   ptychography_base.py:934-947, 970-989
+//   descan ramp on the exit wave      ptychography_base.py:903-905
+//   far field, mode sum, fftshift     detector_models.py:22-30
+//   loss                              ptychography_base.py:908-932
+//   gradient                          torch autograd of the above (ptychography.py:327-329)
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "fft2.h"
+#include "qem_ptycho.h"
+
+namespace qem {
+
+struct Params {
+  int S, M, B, H, W, P, loss_type;
+  float mean_I;
+  unsigned flags;
+  const cf* obj;
+  const cf* probe;
+  const float* pos;
+  const cf* prop;
+  const float* targets;
+  const float* mask;
+  const float* descan;
+  float* loss_out;
+  float* intens_out;
+  const float* dloss;
+  cf* grad_obj;
+  cf* grad_probe;
+  float* grad_pos;
+  float* grad_descan;
+  // workspace
+  cf* psi;          // (B*M*S, N, N)  incident waves; reused by the adjoint for per-mode object-gradient patches
+  cf* X;            // (B*M, N, N)    far-field waves; reused by the adjoint for probe-gradient spectra
+  float* gI;        // (B, N, N)      dL/dI, unshifted
+  double* loss_part;  // per loss block
+  cf* pk;           // (M, N, N)      FFT2 of the probe modes
+  cf* gsum;         // (M, N, N)      sum over the batch of probe-gradient spectra
+  float* gpos_part;   // (B*M, 2)
+  float* gdesc_part;  // (B*M, 2)
+};
+
+__device__ __forceinline__ int pmod(int a, int n) {
+  int r = a % n;
+  return r < 0 ? r + n : r;
+}
+
+// numpy/torch fftfreq(N, d=1)[k] in float32 (exact for power-of-two N).
+template <int N>
+__device__ __forceinline__ float fftfreq1(int k) {
+  return (float)(k < N / 2 ? k : k - N) * (1.0f / (float)N);
+}
+
+// exp(-2 pi i x) for float32 x.
+__device__ __forceinline__ cf expm2pi(float x) {
+  float s, c;
+  sincospif(-2.0f * x, &s, &c);
+  return mk(c, s);
+}
+
+// Maps a linear block index to a work item so that consecutive items land on the same XCD
+// (blocks are dispatched round-robin over the 8 XCDs; each XCD has its own L2).
+__device__ __forceinline__ int xcd_item(int blk, int nblk) {
+  if ((nblk & 7) != 0) return blk;
+  return (blk & 7) * (nblk >> 3) + (blk >> 3);
+}
+
+template <int N>
+__device__ __forceinline__ void origin_and_shift(const float* pos, int b, int& r0, int& c0, float& dr, float& dc) {
+  const float pr = pos[2 * b], pc = pos[2 * b + 1];
+  const float rr = rintf(pr), rc = rintf(pc);  // round half to even, as torch.round
+  r0 = (int)rr;
+  c0 = (int)rc;
+  dr = pr - rr;
+  dc = pc - rc;
+}
+
+// Block-wide sum of one double per thread; result valid in thread 0.
+template <int T>
+__device__ __forceinline__ double block_sum(double x, double* red) {
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) red[w] = x;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < T / 64; ++i) t += red[i];
+  }
+  __syncthreads();
+  return t;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Probe spectra: pk[m] = FFT2(probe[m]).
+template <int N>
+__global__ __launch_bounds__(Fft2<N>::T) void k_probe_spectrum(Params p) {
+  __shared__ cf lds[Fft2<N>::LDS];
+  using F = Fft2<N>;
+  F f(threadIdx.x);
+  const int m = blockIdx.x;
+  cf v[F::PTS];
+  const cf* src = p.probe + (size_t)m * N * N;
+#pragma unroll
+  for (int e = 0; e < F::PTS; ++e) v[e] = src[f.p1_off(e)];
+  f.fwd(v, lds);
+  cf* dst = p.pk + (size_t)m * N * N;
+#pragma unroll
+  for (int e = 0; e < F::PTS; ++e) dst[f.p3_off(e)] = v[e];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Forward multislice for one (pattern, mode): psi_0 = ifft2(fft2(P_m) * ramp_b);
+// for s: x_s = O_s[patch] * psi_s; psi_{s+1} = ifft2(fft2(x_s) * prop_s); X = fft2_ortho(x_{S-1}).
+template <int N, bool DESCAN>
+__global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
+  __shared__ cf lds[Fft2<N>::LDS];
+  using F = Fft2<N>;
+  constexpr int PTS = F::PTS;
+  F f(threadIdx.x);
+  const int item = xcd_item(blockIdx.x, gridDim.x);
+  const int b = item / p.M, m = item - b * p.M;
+  int r0, c0;
+  float dr, dc;
+  origin_and_shift<N>(p.pos, b, r0, c0, dr, dc);
+
+  cf v[PTS];
+  // shifted probe: spectrum * ramp / N^2, inverse transform
+  {
+    const cf* pk = p.pk + (size_t)m * N * N;
+    const cf rc = expm2pi(fftfreq1<N>(f.kc) * dc);
+    const float inv_n2 = 1.0f / (float)(N * N);
+#pragma unroll
+    for (int e = 0; e < PTS; ++e) {
+      const int kr = f.p3_row(e);
+      const cf ramp = cmul(expm2pi(fftfreq1<N>(kr) * dr), rc);
+      v[e] = cscale(cmul(pk[kr * N + f.kc], ramp), inv_n2);
+    }
+  }
+  f.inv(v, lds);
+
+  // object rows / cols of this thread's elements (periodic wrap, fftfreq offsets)
+  int orow[F::RR1], ocol[F::RC1];
+#pragma unroll
+  for (int i = 0; i < F::RR1; ++i) {
+    const int r = i * F::NRP + f.rp;
+    orow[i] = pmod(r0 + (r < N / 2 ? r : r - N), p.H) * p.W;
+  }
+#pragma unroll
+  for (int i = 0; i < F::RC1; ++i) {
+    const int c = i * F::RC2 + f.c0;
+    ocol[i] = pmod(c0 + (c < N / 2 ? c : c - N), p.W);
+  }
+
+  const bool store = (p.flags & QEM_FLAG_STORE_WAVES) != 0;
+  const size_t HW = (size_t)p.H * p.W;
+  const float inv_n2 = 1.0f / (float)(N * N);
+  for (int s = 0; s < p.S; ++s) {
+    const cf* O = p.obj + (size_t)s * HW;
+    if (store) {
+      cf* dst = p.psi + ((size_t)item * p.S + s) * N * N;
+#pragma unroll
+      for (int e = 0; e < PTS; ++e) dst[f.p1_off(e)] = v[e];
+    }
+#pragma unroll
+    for (int e = 0; e < PTS; ++e) v[e] = cmul(O[orow[e / F::RC1] + ocol[e % F::RC1]], v[e]);
+    if (s + 1 < p.S) {
+      f.fwd(v, lds);
+      const cf* pr = p.prop + (size_t)s * N * N;
+#pragma unroll
+      for (int e = 0; e < PTS; ++e) v[e] = cscale(cmul(v[e], pr[f.p3_off(e)]), inv_n2);
+      f.inv(v, lds);
+    }
+  }
+  if constexpr (DESCAN) {
+    const float ddr = p.descan[2 * b], ddc = p.descan[2 * b + 1];
+#pragma unroll
+    for (int e = 0; e < PTS; ++e) {
+      const cf ramp = cmul(expm2pi(fftfreq1<N>(f.p1_row(e)) * ddr), expm2pi(fftfreq1<N>(f.p1_col(e)) * ddc));
+      v[e] = cmul(v[e], ramp);
+    }
+  }
+  f.fwd(v, lds);
+  cf* X = p.X + (size_t)item * N * N;
+  const float inv_n = 1.0f / (float)N;
+#pragma unroll
+  for (int e = 0; e < PTS; ++e) X[f.p3_off(e)] = cscale(v[e], inv_n);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Loss: I = sum_m |X_m|^2 (unshifted k), compared with fftshifted targets; writes dL/dI and
+// per-block partial sums.  256 threads per block, one frequency pixel per thread.
+template <int N>
+__global__ __launch_bounds__(256) void k_loss(Params p) {
+  __shared__ double red[4];
+  const int idx = blockIdx.x * 256 + threadIdx.x;  // over B*N*N (multiple of 256)
+  const int b = idx / (N * N);
+  const int k = idx - b * N * N;
+  const int kr = k / N, kc = k - kr * N;
+  float I = 0.0f;
+  for (int m = 0; m < p.M; ++m) I += cabs2(p.X[((size_t)b * p.M + m) * N * N + k]);
+  const int ks = ((kr + N / 2) & (N - 1)) * N + ((kc + N / 2) & (N - 1));
+  if (p.intens_out != nullptr) p.intens_out[(size_t)b * N * N + ks] = I;
+  const float t = p.targets[(size_t)b * N * N + ks];
+  const bool amp = (p.loss_type == QEM_LOSS_L2_AMPLITUDE || p.loss_type == QEM_LOSS_L1_AMPLITUDE);
+  const float pred = amp ? sqrtf(I + 1e-9f) : I;
+  float term, dpred;
+  if (p.loss_type == QEM_LOSS_POISSON) {
+    term = pred - t * logf(pred + 1e-6f);
+    dpred = 1.0f - t / (pred + 1e-6f);
+  } else {
+    const float mk_ = p.mask[ks];
+    const float d = pred * mk_ - t * mk_;
+    if (p.loss_type == QEM_LOSS_L2_AMPLITUDE || p.loss_type == QEM_LOSS_L2_INTENSITY) {
+      term = d * d;
+      dpred = 2.0f * d * mk_;
+    } else {
+      term = fabsf(d);
+      dpred = (d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f)) * mk_;
+    }
+  }
+  const float scale = (p.loss_type == QEM_LOSS_POISSON) ? 1.0f / p.mean_I
+                                                          : (float)p.P / ((float)p.B * p.mean_I);
+  const float dI = amp ? dpred * 0.5f / pred : dpred;
+  p.gI[(size_t)b * N * N + k] = scale * dI;
+  const double tot = block_sum<256>((double)term, red);
+  if (threadIdx.x == 0) p.loss_part[blockIdx.x] = tot;
+}
+
+// Fixed-order sum of the loss partials.
+__global__ __launch_bounds__(256) void k_loss_finalize(Params p, int nparts) {
+  __shared__ double red[4];
+  double x = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += 256) x += p.loss_part[i];
+  const double tot = block_sum<256>(x, red);
+  if (threadIdx.x == 0) {
+    const double scale = (p.loss_type == QEM_LOSS_POISSON) ? 1.0 / (double)p.mean_I
+                                                            : (double)p.P / ((double)p.B * (double)p.mean_I);
+    p.loss_out[0] = (float)(tot * scale);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Adjoint for one (pattern, mode).  g_X = 2 dL/dI X; g_x = ifft2_ortho^H(g_X); then for s = S-1..0:
+//   dO_s[patch] (this mode) = conj(psi_s) g_x      -> stored over psi_s (natural order)
+//   g_psi = conj(O_s) g_x;  g_x = ifft2(fft2(g_psi) conj(prop_{s-1}))   (s > 0)
+// At s = 0: G = fft2(g_psi_0) and the probe-gradient spectrum conj(ramp_b) G is stored over X;
+// the position gradient reduces Im(conj(G) P_m ramp_b) against the frequencies.
+template <int N, bool DESCAN>
+__global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
+  __shared__ cf lds[Fft2<N>::LDS];
+  __shared__ double red[Fft2<N>::T / 64];
+  using F = Fft2<N>;
+  constexpr int PTS = F::PTS;
+  F f(threadIdx.x);
+  const int item = xcd_item(blockIdx.x, gridDim.x);
+  const int b = item / p.M, m = item - b * p.M;
+  int r0, c0;
+  float dr, dc;
+  origin_and_shift<N>(p.pos, b, r0, c0, dr, dc);
+  const float dl = p.dloss[0];
+  const float inv_n = 1.0f / (float)N;
+  const float inv_n2 = 1.0f / (float)(N * N);
+
+  cf v[PTS];
+  cf* Xs = p.X + (size_t)item * N * N;
+  {
+    const float* gI = p.gI + (size_t)b * N * N;
+    const float sc = 2.0f * dl * inv_n;
+#pragma unroll
+    for (int e = 0; e < PTS; ++e) {
+      const int k = f.p3_off(e);
+      v[e] = cscale(Xs[k], sc * gI[k]);
+    }
+  }
+  f.inv(v, lds);
+
+  int orow[F::RR1], ocol[F::RC1];
+#pragma unroll
+  for (int i = 0; i < F::RR1; ++i) {
+    const int r = i * F::NRP + f.rp;
+    orow[i] = pmod(r0 + (r < N / 2 ? r : r - N), p.H) * p.W;
+  }
+#pragma unroll
+  for (int i = 0; i < F::RC1; ++i) {
+    const int c = i * F::RC2 + f.c0;
+    ocol[i] = pmod(c0 + (c < N / 2 ? c : c - N), p.W);
+  }
+
+  const size_t HW = (size_t)p.H * p.W;
+  const bool want_desc = DESCAN && (p.flags & QEM_FLAG_GRAD_DESCAN);
+  // Descan (applied after the last slice in the forward): g_x <- conj(ramp) g_x, and its gradient
+  // 2 pi sum f * Im(conj(g_x) x_out) with x_out = ramp * O_{S-1} psi_{S-1}.
+  if constexpr (DESCAN) {
+    const cf* O = p.obj + (size_t)(p.S - 1) * HW;
+    const cf* ps = p.psi + ((size_t)item * p.S + p.S - 1) * N * N;
+    const float ddr = p.descan[2 * b], ddc = p.descan[2 * b + 1];
+    double ar = 0.0, ac = 0.0;
+#pragma unroll
+    for (int e = 0; e < PTS; ++e) {
+      const int row = f.p1_row(e), col = f.p1_col(e);
+      const float fr = fftfreq1<N>(row), fc = fftfreq1<N>(col);
+      const cf ramp = cmul(expm2pi(fr * ddr), expm2pi(fc * ddc));
+      if (want_desc) {
+        const cf xo = cmul(cmul(O[orow[e / F::RC1] + ocol[e % F::RC1]], ps[f.p1_off(e)]), ramp);
+        const float im = cconjmul(v[e], xo).y;
+        ar += (double)(fr * im);
+        ac += (double)(fc * im);
+      }
+      v[e] = cmulc(v[e], ramp);
+    }
+    if (want_desc) {
+      const double tr = block_sum<F::T>(ar, red);
+      const double tc = block_sum<F::T>(ac, red);
+      if (threadIdx.x == 0) {
+        p.gdesc_part[2 * item] = (float)(2.0 * M_PI * tr);
+        p.gdesc_part[2 * item + 1] = (float)(2.0 * M_PI * tc);
+      }
+    }
+  }
+  for (int s = p.S - 1; s >= 0; --s) {
+    const cf* O = p.obj + (size_t)s * HW;
+    cf* ps = p.psi + ((size_t)item * p.S + s) * N * N;
+#pragma unroll
+    for (int e = 0; e < PTS; ++e) {
+      const int a = f.p1_off(e);
+      const cf psi = ps[a];
+      const cf o = O[orow[e / F::RC1] + ocol[e % F::RC1]];
+      ps[a] = cconjmul(psi, v[e]);
+      v[e] = cconjmul(o, v[e]);
+    }
+    f.fwd(v, lds);
+    if (s > 0) {
+      const cf* pr = p.prop + (size_t)(s - 1) * N * N;
+#pragma unroll
+      for (int e = 0; e < PTS; ++e) v[e] = cscale(cmulc(v[e], pr[f.p3_off(e)]), inv_n2);
+      f.inv(v, lds);
+    }
+  }
+  // v = G = fft2(g_psi_0) in P3.
+  const cf rc = expm2pi(fftfreq1<N>(f.kc) * dc);
+  const cf* pk = p.pk + (size_t)m * N * N;
+  const bool want_pos = (p.flags & QEM_FLAG_GRAD_POSITIONS) && (p.gpos_part != nullptr);
+  double ar = 0.0, ac = 0.0;
+  const float fc = fftfreq1<N>(f.kc);
+#pragma unroll
+  for (int e = 0; e < PTS; ++e) {
+    const int kr = f.p3_row(e);
+    const float fr = fftfreq1<N>(kr);
+    const cf ramp = cmul(expm2pi(fr * dr), rc);
+    const int k = f.p3_off(e);
+    Xs[k] = cmulc(v[e], ramp);
+    if (want_pos) {
+      const float im = cconjmul(v[e], cmul(pk[k], ramp)).y;
+      ar += (double)(fr * im);
+      ac += (double)(fc * im);
+    }
+  }
+  if (want_pos) {
+    const double tr = block_sum<F::T>(ar, red);
+    const double tc = block_sum<F::T>(ac, red);
+    if (threadIdx.x == 0) {
+      p.gpos_part[2 * item] = (float)(2.0 * M_PI * tr * (double)inv_n2);
+      p.gpos_part[2 * item + 1] = (float)(2.0 * M_PI * tc * (double)inv_n2);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Deterministic object-gradient scatter-add: each block owns a 32x32 tile of one slice and sums,
+// in increasing (b, m) order, the per-mode patch gradients that land on it.
+template <int N>
+__global__ __launch_bounds__(256) void k_obj_reduce(Params p) {
+  const int s = blockIdx.z;
+  const int x = blockIdx.x * 32 + (threadIdx.x & 31);
+  const int ybase = blockIdx.y * 32 + (threadIdx.x >> 5);
+  cf acc[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) acc[j] = mk(0.0f, 0.0f);
+  const bool xin = x < p.W;
+  for (int b = 0; b < p.B; ++b) {
+    const float pr = p.pos[2 * b], pc = p.pos[2 * b + 1];
+    const int r0 = (int)rintf(pr), c0 = (int)rintf(pc);
+    const int ux = pmod(x - c0 + N / 2, p.W);
+    if (!xin || ux >= N) continue;
+    const int col = (ux + N / 2) & (N - 1);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int y = ybase + 8 * j;
+      const int uy = pmod(y - r0 + N / 2, p.H);
+      if (y < p.H && uy < N) {
+        const int row = (uy + N / 2) & (N - 1);
+        const cf* g = p.psi + (((size_t)b * p.M) * p.S + s) * N * N + row * N + col;
+        for (int m = 0; m < p.M; ++m) {
+          const cf t = g[(size_t)m * p.S * N * N];
+          acc[j].x += t.x;
+          acc[j].y += t.y;
+        }
+      }
+    }
+  }
+  if (!xin) return;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int y = ybase + 8 * j;
+    if (y < p.H) p.grad_obj[(size_t)s * p.H * p.W + (size_t)y * p.W + x] = acc[j];
+  }
+}
+
+// gsum[m][k] = sum_b spectra[b][m][k], fixed order.
+template <int N>
+__global__ __launch_bounds__(256) void k_probe_reduce(Params p) {
+  const int idx = blockIdx.x * 256 + threadIdx.x;  // over M*N*N
+  const int m = idx / (N * N);
+  const int k = idx - m * N * N;
+  cf acc = mk(0.0f, 0.0f);
+  for (int b = 0; b < p.B; ++b) {
+    const cf t = p.X[((size_t)b * p.M + m) * N * N + k];
+    acc.x += t.x;
+    acc.y += t.y;
+  }
+  p.gsum[(size_t)m * N * N + k] = acc;
+}
+
+// grad_probe[m] = ifft2(gsum[m]) (1/N^2 included).
+template <int N>
+__global__ __launch_bounds__(Fft2<N>::T) void k_probe_grad(Params p) {
+  __shared__ cf lds[Fft2<N>::LDS];
+  using F = Fft2<N>;
+  F f(threadIdx.x);
+  const int m = blockIdx.x;
+  cf v[F::PTS];
+  const cf* src = p.gsum + (size_t)m * N * N;
+#pragma unroll
+  for (int e = 0; e < F::PTS; ++e) v[e] = src[f.p3_off(e)];
+  f.inv(v, lds);
+  const float inv_n2 = 1.0f / (float)(N * N);
+  cf* dst = p.grad_probe + (size_t)m * N * N;
+#pragma unroll
+  for (int e = 0; e < F::PTS; ++e) dst[f.p1_off(e)] = cscale(v[e], inv_n2);
+}
+
+// grad[b][i] = sum_m part[(b*M + m)*2 + i]
+__global__ __launch_bounds__(256) void k_pair_reduce(const float* part, float* out, int B, int M) {
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx >= 2 * B) return;
+  const int b = idx >> 1, i = idx & 1;
+  float acc = 0.0f;
+  for (int m = 0; m < M; ++m) acc += part[((size_t)b * M + m) * 2 + i];
+  out[idx] = acc;
+}
+
+// Batched 2-D FFT utility (torch default norms).
+template <int N>
+__global__ __launch_bounds__(Fft2<N>::T) void k_fft2(const cf* in, cf* out, int inverse) {
+  __shared__ cf lds[Fft2<N>::LDS];
+  using F = Fft2<N>;
+  F f(threadIdx.x);
+  const size_t base = (size_t)blockIdx.x * N * N;
+  cf v[F::PTS];
+  if (!inverse) {
+#pragma unroll
+    for (int e = 0; e < F::PTS; ++e) v[e] = in[base + f.p1_off(e)];
+    f.fwd(v, lds);
+#pragma unroll
+    for (int e = 0; e < F::PTS; ++e) out[base + f.p3_off(e)] = v[e];
+  } else {
+#pragma unroll
+    for (int e = 0; e < F::PTS; ++e) v[e] = in[base + f.p3_off(e)];
+    f.inv(v, lds);
+    const float s = 1.0f / (float)(N * N);
+#pragma unroll
+    for (int e = 0; e < F::PTS; ++e) out[base + f.p1_off(e)] = cscale(v[e], s);
+  }
+}
+
+}  // namespace qem
+
+// =============================================================================================
+// C ABI
+// =============================================================================================
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+inline size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Optional event pairs recorded around the dominant kernels (k_forward, k_adjoint) for timing.
+struct Profile {
+  bool on = false;
+  int cap = 0;
+  int nf = 0, na = 0;
+  hipEvent_t* fwd = nullptr;  // 2 * cap events
+  hipEvent_t* adj = nullptr;
+};
+Profile g_prof;
+
+void prof_mark(bool adjoint_kernel, int edge, hipStream_t st) {
+  if (!g_prof.on) return;
+  int& n = adjoint_kernel ? g_prof.na : g_prof.nf;
+  hipEvent_t* ev = adjoint_kernel ? g_prof.adj : g_prof.fwd;
+  if (n >= g_prof.cap) return;
+  hipEventRecord(ev[2 * n + edge], st);
+  if (edge == 1) ++n;
+}
+
+struct Layout {
+  size_t psi, X, gI, loss_part, pk, gsum, gpos, gdesc, total;
+  int nloss;
+};
+
+Layout layout_of(const qem_ptycho_desc& d) {
+  Layout L;
+  const size_t N2 = (size_t)d.roi * d.roi;
+  const size_t BM = (size_t)d.batch * d.num_modes;
+  size_t o = 0;
+  L.psi = o;
+  if (d.flags & QEM_FLAG_STORE_WAVES) o += align_up(BM * d.num_slices * N2 * 8);
+  L.X = o;
+  o += align_up(BM * N2 * 8);
+  L.gI = o;
+  o += align_up((size_t)d.batch * N2 * 4);
+  L.nloss = (int)((size_t)d.batch * N2 / 256);
+  L.loss_part = o;
+  o += align_up((size_t)L.nloss * 8);
+  L.pk = o;
+  o += align_up((size_t)d.num_modes * N2 * 8);
+  L.gsum = o;
+  o += align_up((size_t)d.num_modes * N2 * 8);
+  L.gpos = o;
+  o += align_up(BM * 2 * 4);
+  L.gdesc = o;
+  o += align_up(BM * 2 * 4);
+  L.total = o;
+  return L;
+}
+
+int check_desc(const qem_ptycho_desc* d) {
+  if (d == nullptr) return fail(QEM_ERR_NULL, "desc is NULL");
+  if (d->roi != 32 && d->roi != 64 && d->roi != 128)
+    return fail(QEM_ERR_UNSUPPORTED, "roi %d not supported (32, 64 or 128)", d->roi);
+  if (d->num_slices < 1) return fail(QEM_ERR_ARG, "num_slices must be >= 1 (got %d)", d->num_slices);
+  if (d->num_modes < 1) return fail(QEM_ERR_ARG, "num_modes must be >= 1 (got %d)", d->num_modes);
+  if (d->batch < 1) return fail(QEM_ERR_ARG, "batch must be >= 1 (got %d)", d->batch);
+  if (d->obj_rows < d->roi || d->obj_cols < d->roi)
+    return fail(QEM_ERR_ARG, "object (%d, %d) must be at least roi %d in each dimension", d->obj_rows, d->obj_cols,
+                d->roi);
+  if ((size_t)d->num_slices * d->obj_rows * d->obj_cols >= ((size_t)1 << 31))
+    return fail(QEM_ERR_ARG, "object has too many elements");
+  if (d->num_gpts < 1) return fail(QEM_ERR_ARG, "num_gpts must be >= 1 (got %d)", d->num_gpts);
+  if (d->loss_type < QEM_LOSS_L2_AMPLITUDE || d->loss_type > QEM_LOSS_POISSON)
+    return fail(QEM_ERR_ARG, "unknown loss_type %d", d->loss_type);
+  if (!(d->mean_intensity > 0.0f) || !isfinite(d->mean_intensity))
+    return fail(QEM_ERR_ARG, "mean_intensity must be finite and > 0");
+  return QEM_OK;
+}
+
+int check_ptr(const void* ptr, const char* name, bool complex_) {
+  if (ptr == nullptr) return fail(QEM_ERR_NULL, "%s is NULL", name);
+  if (((uintptr_t)ptr) & (complex_ ? 7u : 3u)) return fail(QEM_ERR_ALIGN, "%s is misaligned", name);
+  return QEM_OK;
+}
+
+int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(QEM_ERR_LAUNCH, "%s launch failed: %s", what, hipGetErrorString(e));
+  return QEM_OK;
+}
+
+qem::Params make_params(const qem_ptycho_desc& d, const Layout& L, void* ws) {
+  qem::Params p;
+  memset(&p, 0, sizeof(p));
+  p.S = d.num_slices;
+  p.M = d.num_modes;
+  p.B = d.batch;
+  p.H = d.obj_rows;
+  p.W = d.obj_cols;
+  p.P = d.num_gpts;
+  p.loss_type = d.loss_type;
+  p.mean_I = d.mean_intensity;
+  p.flags = d.flags;
+  char* w = (char*)ws;
+  p.psi = (d.flags & QEM_FLAG_STORE_WAVES) ? (qem::cf*)(w + L.psi) : nullptr;
+  p.X = (qem::cf*)(w + L.X);
+  p.gI = (float*)(w + L.gI);
+  p.loss_part = (double*)(w + L.loss_part);
+  p.pk = (qem::cf*)(w + L.pk);
+  p.gsum = (qem::cf*)(w + L.gsum);
+  p.gpos_part = (float*)(w + L.gpos);
+  p.gdesc_part = (float*)(w + L.gdesc);
+  return p;
+}
+
+template <int N>
+int launch_forward(const qem::Params& p, const Layout& L, hipStream_t st) {
+  using F = qem::Fft2<N>;
+  hipLaunchKernelGGL(qem::k_probe_spectrum<N>, dim3(p.M), dim3(F::T), 0, st, p);
+  if (int rc = check_launch("probe_spectrum")) return rc;
+  prof_mark(false, 0, st);
+  if (p.descan != nullptr)
+    hipLaunchKernelGGL((qem::k_forward<N, true>), dim3(p.B * p.M), dim3(F::T), 0, st, p);
+  else
+    hipLaunchKernelGGL((qem::k_forward<N, false>), dim3(p.B * p.M), dim3(F::T), 0, st, p);
+  prof_mark(false, 1, st);
+  if (int rc = check_launch("forward")) return rc;
+  hipLaunchKernelGGL(qem::k_loss<N>, dim3(L.nloss), dim3(256), 0, st, p);
+  if (int rc = check_launch("loss")) return rc;
+  hipLaunchKernelGGL(qem::k_loss_finalize, dim3(1), dim3(256), 0, st, p, L.nloss);
+  return check_launch("loss_finalize");
+}
+
+template <int N>
+int launch_adjoint(const qem::Params& p, hipStream_t st) {
+  using F = qem::Fft2<N>;
+  prof_mark(true, 0, st);
+  if (p.descan != nullptr)
+    hipLaunchKernelGGL((qem::k_adjoint<N, true>), dim3(p.B * p.M), dim3(F::T), 0, st, p);
+  else
+    hipLaunchKernelGGL((qem::k_adjoint<N, false>), dim3(p.B * p.M), dim3(F::T), 0, st, p);
+  prof_mark(true, 1, st);
+  if (int rc = check_launch("adjoint")) return rc;
+  dim3 g((p.W + 31) / 32, (p.H + 31) / 32, p.S);
+  hipLaunchKernelGGL(qem::k_obj_reduce<N>, g, dim3(256), 0, st, p);
+  if (int rc = check_launch("obj_reduce")) return rc;
+  hipLaunchKernelGGL(qem::k_probe_reduce<N>, dim3(p.M * N * N / 256), dim3(256), 0, st, p);
+  if (int rc = check_launch("probe_reduce")) return rc;
+  hipLaunchKernelGGL(qem::k_probe_grad<N>, dim3(p.M), dim3(F::T), 0, st, p);
+  if (int rc = check_launch("probe_grad")) return rc;
+  if (p.grad_pos != nullptr) {
+    hipLaunchKernelGGL(qem::k_pair_reduce, dim3((2 * p.B + 255) / 256), dim3(256), 0, st, p.gpos_part, p.grad_pos,
+                       p.B, p.M);
+    if (int rc = check_launch("position_reduce")) return rc;
+  }
+  if (p.grad_descan != nullptr) {
+    hipLaunchKernelGGL(qem::k_pair_reduce, dim3((2 * p.B + 255) / 256), dim3(256), 0, st, p.gdesc_part,
+                       p.grad_descan, p.B, p.M);
+    if (int rc = check_launch("descan_reduce")) return rc;
+  }
+  return QEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t qem_abi_version(void) { return 100; }
+
+int qem_profile_begin(int32_t max_launches) {
+  g_err[0] = 0;
+  if (g_prof.on) return fail(QEM_ERR_ARG, "profiling already active");
+  if (max_launches < 1 || max_launches > 100000) return fail(QEM_ERR_ARG, "max_launches out of range");
+  g_prof.cap = max_launches;
+  g_prof.nf = g_prof.na = 0;
+  g_prof.fwd = new hipEvent_t[2 * max_launches];
+  g_prof.adj = new hipEvent_t[2 * max_launches];
+  for (int i = 0; i < 2 * max_launches; ++i) {
+    if (hipEventCreate(&g_prof.fwd[i]) != hipSuccess || hipEventCreate(&g_prof.adj[i]) != hipSuccess)
+      return fail(QEM_ERR_LAUNCH, "hipEventCreate failed");
+  }
+  g_prof.on = true;
+  return QEM_OK;
+}
+
+int qem_profile_end(float* fwd_ms, int32_t* n_fwd, float* adj_ms, int32_t* n_adj) {
+  g_err[0] = 0;
+  if (!g_prof.on) return fail(QEM_ERR_ARG, "profiling not active");
+  float tf = 0.f, ta = 0.f;
+  int rc = QEM_OK;
+  for (int i = 0; i < g_prof.nf && rc == QEM_OK; ++i) {
+    float ms = 0.f;
+    if (hipEventSynchronize(g_prof.fwd[2 * i + 1]) != hipSuccess ||
+        hipEventElapsedTime(&ms, g_prof.fwd[2 * i], g_prof.fwd[2 * i + 1]) != hipSuccess)
+      rc = fail(QEM_ERR_LAUNCH, "event timing failed");
+    tf += ms;
+  }
+  for (int i = 0; i < g_prof.na && rc == QEM_OK; ++i) {
+    float ms = 0.f;
+    if (hipEventSynchronize(g_prof.adj[2 * i + 1]) != hipSuccess ||
+        hipEventElapsedTime(&ms, g_prof.adj[2 * i], g_prof.adj[2 * i + 1]) != hipSuccess)
+      rc = fail(QEM_ERR_LAUNCH, "event timing failed");
+    ta += ms;
+  }
+  if (fwd_ms) *fwd_ms = tf;
+  if (adj_ms) *adj_ms = ta;
+  if (n_fwd) *n_fwd = g_prof.nf;
+  if (n_adj) *n_adj = g_prof.na;
+  for (int i = 0; i < 2 * g_prof.cap; ++i) {
+    hipEventDestroy(g_prof.fwd[i]);
+    hipEventDestroy(g_prof.adj[i]);
+  }
+  delete[] g_prof.fwd;
+  delete[] g_prof.adj;
+  g_prof = Profile();
+  return rc;
+}
+
+const char* qem_last_error(void) { return g_err; }
+
+int qem_ptycho_workspace_bytes(const qem_ptycho_desc* desc, size_t* bytes) {
+  g_err[0] = 0;
+  if (int rc = check_desc(desc)) return rc;
+  if (bytes == nullptr) return fail(QEM_ERR_NULL, "bytes is NULL");
+  *bytes = layout_of(*desc).total;
+  return QEM_OK;
+}
+
+int qem_ptycho_forward_loss(const qem_ptycho_desc* desc, const void* obj, const void* probe, const float* positions,
+                            const void* propagators, const float* targets, const float* mask, const float* descan,
+                            float* loss, float* intensities, void* workspace, size_t workspace_bytes, void* stream) {
+  g_err[0] = 0;
+  if (int rc = check_desc(desc)) return rc;
+  const qem_ptycho_desc& d = *desc;
+  if (int rc = check_ptr(obj, "obj", true)) return rc;
+  if (int rc = check_ptr(probe, "probe", true)) return rc;
+  if (int rc = check_ptr(positions, "positions", false)) return rc;
+  if (d.num_slices > 1)
+    if (int rc = check_ptr(propagators, "propagators", true)) return rc;
+  if (int rc = check_ptr(targets, "targets", false)) return rc;
+  if (d.loss_type != QEM_LOSS_POISSON)
+    if (int rc = check_ptr(mask, "mask", false)) return rc;
+  if (descan != nullptr)
+    if (int rc = check_ptr(descan, "descan", false)) return rc;
+  if (int rc = check_ptr(loss, "loss", false)) return rc;
+  if (intensities != nullptr)
+    if (int rc = check_ptr(intensities, "intensities", false)) return rc;
+  if (int rc = check_ptr(workspace, "workspace", true)) return rc;
+  const Layout L = layout_of(d);
+  if (workspace_bytes < L.total)
+    return fail(QEM_ERR_WORKSPACE, "workspace has %zu bytes, needs %zu", workspace_bytes, L.total);
+  qem::Params p = make_params(d, L, workspace);
+  p.obj = (const qem::cf*)obj;
+  p.probe = (const qem::cf*)probe;
+  p.pos = positions;
+  p.prop = (const qem::cf*)propagators;
+  p.targets = targets;
+  p.mask = mask;
+  p.descan = descan;
+  p.loss_out = loss;
+  p.intens_out = intensities;
+  hipStream_t st = (hipStream_t)stream;
+  switch (d.roi) {
+    case 32: return launch_forward<32>(p, L, st);
+    case 64: return launch_forward<64>(p, L, st);
+    default: return launch_forward<128>(p, L, st);
+  }
+}
+
+int qem_ptycho_adjoint(const qem_ptycho_desc* desc, const void* obj, const void* probe, const float* positions,
+                       const void* propagators, const float* descan, const float* dloss, void* grad_obj,
+                       void* grad_probe, float* grad_positions, float* grad_descan, void* workspace,
+                       size_t workspace_bytes, void* stream) {
+  g_err[0] = 0;
+  if (int rc = check_desc(desc)) return rc;
+  const qem_ptycho_desc& d = *desc;
+  if (!(d.flags & QEM_FLAG_STORE_WAVES))
+    return fail(QEM_ERR_ARG, "adjoint needs a forward pass run with QEM_FLAG_STORE_WAVES");
+  if (int rc = check_ptr(obj, "obj", true)) return rc;
+  if (int rc = check_ptr(probe, "probe", true)) return rc;
+  if (int rc = check_ptr(positions, "positions", false)) return rc;
+  if (d.num_slices > 1)
+    if (int rc = check_ptr(propagators, "propagators", true)) return rc;
+  if (descan != nullptr)
+    if (int rc = check_ptr(descan, "descan", false)) return rc;
+  if (int rc = check_ptr(dloss, "dloss", false)) return rc;
+  if (int rc = check_ptr(grad_obj, "grad_obj", true)) return rc;
+  if (int rc = check_ptr(grad_probe, "grad_probe", true)) return rc;
+  if (grad_positions != nullptr) {
+    if (!(d.flags & QEM_FLAG_GRAD_POSITIONS))
+      return fail(QEM_ERR_ARG, "grad_positions given without QEM_FLAG_GRAD_POSITIONS");
+    if (int rc = check_ptr(grad_positions, "grad_positions", false)) return rc;
+  }
+  if (grad_descan != nullptr) {
+    if (!(d.flags & QEM_FLAG_GRAD_DESCAN) || descan == nullptr)
+      return fail(QEM_ERR_ARG, "grad_descan needs QEM_FLAG_GRAD_DESCAN and a descan input");
+    if (int rc = check_ptr(grad_descan, "grad_descan", false)) return rc;
+  }
+  if (int rc = check_ptr(workspace, "workspace", true)) return rc;
+  const Layout L = layout_of(d);
+  if (workspace_bytes < L.total)
+    return fail(QEM_ERR_WORKSPACE, "workspace has %zu bytes, needs %zu", workspace_bytes, L.total);
+  qem::Params p = make_params(d, L, workspace);
+  p.obj = (const qem::cf*)obj;
+  p.probe = (const qem::cf*)probe;
+  p.pos = positions;
+  p.prop = (const qem::cf*)propagators;
+  p.descan = descan;
+  p.dloss = dloss;
+  p.grad_obj = (qem::cf*)grad_obj;
+  p.grad_probe = (qem::cf*)grad_probe;
+  p.grad_pos = grad_positions;
+  p.grad_descan = grad_descan;
+  hipStream_t st = (hipStream_t)stream;
+  switch (d.roi) {
+    case 32: return launch_adjoint<32>(p, st);
+    case 64: return launch_adjoint<64>(p, st);
+    default: return launch_adjoint<128>(p, st);
+  }
+}
+
+int qem_fft2_c64(int32_t n, int32_t count, int32_t inverse, const void* in, void* out, void* stream) {
+  g_err[0] = 0;
+  if (n != 32 && n != 64 && n != 128) return fail(QEM_ERR_UNSUPPORTED, "fft size %d not supported", n);
+  if (count < 0) return fail(QEM_ERR_ARG, "count must be >= 0");
+  if (count == 0) return QEM_OK;
+  if (int rc = check_ptr(in, "in", true)) return rc;
+  if (int rc = check_ptr(out, "out", true)) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const qem::cf* i = (const qem::cf*)in;
+  qem::cf* o = (qem::cf*)out;
+  switch (n) {
+    case 32: hipLaunchKernelGGL(qem::k_fft2<32>, dim3(count), dim3(qem::Fft2<32>::T), 0, st, i, o, inverse); break;
+    case 64: hipLaunchKernelGGL(qem::k_fft2<64>, dim3(count), dim3(qem::Fft2<64>::T), 0, st, i, o, inverse); break;
+    default: hipLaunchKernelGGL(qem::k_fft2<128>, dim3(count), dim3(qem::Fft2<128>::T), 0, st, i, o, inverse); break;
+  }
+  return check_launch("fft2");
+}
+
+}  // extern "C"

@@ -1,0 +1,255 @@
+"""Drop-in forward/loss/gradient for quantem's multislice mixed-probe ptychography.
+
+The reference runs, per mini-batch (quantem ``ptychography.py:213-245``)::
+
+    patch_indices, _, frac, descan = dset.forward(batch_indices, obj_padding_px)  # dataset_models.py:1482
+    shifted_probes = probe_model.forward(frac)                                     # probe_models.py:718
+    obj_patches = obj_model.forward(patch_indices)                                 # object_models.py:678
+    _, overlap = forward_operator(obj_patches, shifted_probes, descan)             # ptychography_base.py:892
+    pred_intensities = detector_model.forward(overlap)                             # detector_models.py:22
+    loss, targets = error_estimate(pred_intensities, batch_indices, loss_type)     # ptychography_base.py:908
+    loss.backward()                                                                # ptychography.py:328
+
+:func:`multislice_loss` replaces everything from ``probe_model.forward`` to ``loss.backward()``
+with one ``torch.autograd.Function`` whose forward and backward are the HIP kernels of
+``libqem_ptycho.so``.  It consumes the tensors the reference's models expose (``obj_model.obj``
+after hard constraints, ``probe_model.probe`` after orthogonalisation, the dataset's scan
+positions in pixels, the propagators, the gathered targets, the detector mask and the descan
+shifts), so the caller's loop, constraints, optimizers and gradient rescaling stay unchanged.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from typing import Optional
+
+import torch
+
+from . import _native as nat
+
+__all__ = [
+    "multislice_loss",
+    "forward_intensities",
+    "MultisliceLoss",
+    "fft2",
+    "ifft2",
+    "workspace_bytes",
+]
+
+
+def _ptr(t: Optional[torch.Tensor]) -> Optional[int]:
+    return None if t is None else t.data_ptr()
+
+
+def _stream(device: torch.device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def _need(t: torch.Tensor, name: str, dtype: torch.dtype, shape: tuple, device: torch.device) -> torch.Tensor:
+    """Validates like quantem's validate_tensor (core/utils/validators.py:478-495)."""
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch.Tensor, got {type(t).__name__}")
+    if t.dtype != dtype:
+        raise TypeError(f"{name} must have dtype {dtype}, got {t.dtype}")
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name} must have shape {tuple(shape)}, got {tuple(t.shape)}")
+    if t.device != device:
+        raise ValueError(f"{name} must be on {device}, got {t.device}")
+    return t.contiguous()
+
+
+def _desc(S, M, B, N, H, W, num_gpts, loss_type, mean_intensity, flags) -> nat.QemDesc:
+    if loss_type not in nat.LOSS_TYPES:
+        raise ValueError(f"Unknown loss type {loss_type}, should be one of {sorted(nat.LOSS_TYPES)}")
+    return nat.QemDesc(
+        num_slices=S,
+        num_modes=M,
+        batch=B,
+        roi=N,
+        obj_rows=H,
+        obj_cols=W,
+        num_gpts=int(num_gpts),
+        loss_type=nat.LOSS_TYPES[loss_type],
+        mean_intensity=float(mean_intensity),
+        flags=flags,
+    )
+
+
+def workspace_bytes(desc: nat.QemDesc) -> int:
+    lib = nat.load()
+    out = ctypes.c_size_t(0)
+    nat.check(lib.qem_ptycho_workspace_bytes(ctypes.byref(desc), ctypes.byref(out)), "workspace query")
+    return int(out.value)
+
+
+class MultisliceLoss(torch.autograd.Function):
+    """Loss of one batch; backward runs the hand-written adjoint.
+
+    Inputs follow the layouts of the reference tensors (see ``include/qem_ptycho.h``).  Gradients
+    are returned for ``obj``, ``probe``, ``positions`` and ``descan`` (the last two only when they
+    require grad); propagators, targets and mask get ``None`` as in the reference, where they are
+    not optimised.
+    """
+
+    @staticmethod
+    def forward(ctx, obj, probe, positions, propagators, targets, mask, descan, num_gpts, mean_intensity, loss_type,
+                return_intensities, grad_enabled):
+        device = obj.device
+        if device.type != "cuda":
+            raise ValueError(f"obj must be on a GPU device, got {device}")
+        if obj.dim() != 3:
+            raise ValueError(f"obj must have shape (S, H, W), got {tuple(obj.shape)}")
+        if probe.dim() != 3:
+            raise ValueError(f"probe must have shape (M, N, N), got {tuple(probe.shape)}")
+        S, H, W = obj.shape
+        M, N, N2 = probe.shape
+        if N != N2:
+            raise ValueError(f"probe must be square, got {tuple(probe.shape)}")
+        if N not in nat.SUPPORTED_ROI:
+            raise ValueError(f"roi {N} not supported, expected one of {nat.SUPPORTED_ROI}")
+        if positions.dim() != 2:
+            raise ValueError(f"positions must have shape (B, 2), got {tuple(positions.shape)}")
+        B = positions.shape[0]
+        if B < 1:
+            raise ValueError("batch must hold at least one scan position")
+        obj_c = _need(obj, "obj", torch.complex64, (S, H, W), device)
+        probe_c = _need(probe, "probe", torch.complex64, (M, N, N), device)
+        pos_c = _need(positions, "positions", torch.float32, (B, 2), device)
+        if S > 1:
+            prop_c = _need(propagators, "propagators", torch.complex64, (S - 1, N, N), device)
+        else:
+            prop_c = None
+        tgt_c = _need(targets, "targets", torch.float32, (B, N, N), device)
+        if mask is None:
+            mask_c = torch.ones((N, N), dtype=torch.float32, device=device)
+        else:
+            mask_c = _need(mask, "mask", torch.float32, (N, N), device)
+        desc_c = None if descan is None else _need(descan, "descan", torch.float32, (B, 2), device)
+
+        # grad mode is always off inside Function.forward: the caller's mode arrives as grad_enabled
+        need_grad = bool(grad_enabled) and any(ctx.needs_input_grad[i] for i in (0, 1, 2, 6))
+        flags = 0
+        if need_grad:
+            flags |= nat.QEM_FLAG_STORE_WAVES
+            if ctx.needs_input_grad[2]:
+                flags |= nat.QEM_FLAG_GRAD_POSITIONS
+            if desc_c is not None and ctx.needs_input_grad[6]:
+                flags |= nat.QEM_FLAG_GRAD_DESCAN
+        d = _desc(S, M, B, N, H, W, num_gpts, loss_type, mean_intensity, flags)
+        ws = torch.empty(workspace_bytes(d), dtype=torch.uint8, device=device)
+        loss = torch.empty((), dtype=torch.float32, device=device)
+        intens = torch.empty((B, N, N), dtype=torch.float32, device=device) if return_intensities else None
+        lib = nat.load()
+        rc = lib.qem_ptycho_forward_loss(
+            ctypes.byref(d), _ptr(obj_c), _ptr(probe_c), _ptr(pos_c), _ptr(prop_c), _ptr(tgt_c), _ptr(mask_c),
+            _ptr(desc_c), _ptr(loss), _ptr(intens), _ptr(ws), ws.numel(), _stream(device))
+        nat.check(rc, "qem_ptycho_forward_loss")
+        if need_grad:
+            ctx.save_for_backward(obj_c, probe_c, pos_c, prop_c, desc_c, ws)
+            ctx.desc = d
+        if intens is None:
+            intens = torch.empty((0,), dtype=torch.float32, device=device)
+        ctx.mark_non_differentiable(intens)
+        return loss, intens
+
+    @staticmethod
+    def backward(ctx, grad_loss, grad_intens):
+        obj_c, probe_c, pos_c, prop_c, desc_c, ws = ctx.saved_tensors
+        d = ctx.desc
+        device = obj_c.device
+        gl = grad_loss.detach().to(device=device, dtype=torch.float32).contiguous()
+        g_obj = torch.empty_like(obj_c)
+        g_probe = torch.empty_like(probe_c)
+        g_pos = torch.empty_like(pos_c) if (d.flags & nat.QEM_FLAG_GRAD_POSITIONS) else None
+        g_desc = torch.empty_like(desc_c) if (d.flags & nat.QEM_FLAG_GRAD_DESCAN) else None
+        lib = nat.load()
+        rc = lib.qem_ptycho_adjoint(
+            ctypes.byref(d), _ptr(obj_c), _ptr(probe_c), _ptr(pos_c), _ptr(prop_c), _ptr(desc_c), _ptr(gl),
+            _ptr(g_obj), _ptr(g_probe), _ptr(g_pos), _ptr(g_desc), _ptr(ws), ws.numel(), _stream(device))
+        nat.check(rc, "qem_ptycho_adjoint")
+        return (g_obj if ctx.needs_input_grad[0] else None,
+                g_probe if ctx.needs_input_grad[1] else None,
+                g_pos, None, None, None, g_desc, None, None, None, None, None)
+
+
+def multislice_loss(
+    obj: torch.Tensor,
+    probe: torch.Tensor,
+    positions: torch.Tensor,
+    propagators: Optional[torch.Tensor],
+    targets: torch.Tensor,
+    mask: Optional[torch.Tensor] = None,
+    descan: Optional[torch.Tensor] = None,
+    *,
+    num_gpts: int,
+    mean_intensity: float,
+    loss_type: str = "l2_amplitude",
+    return_intensities: bool = False,
+):
+    """Batch loss of quantem's forward model (``ptychography.py:220-229``), differentiable.
+
+    Args:
+        obj: (S, H, W) complex64 transmission after hard constraints (``obj_model.obj``; a real
+            potential V must be passed as ``torch.exp(1j * V)``, ``object_models.py:253-254``).
+        probe: (M, N, N) complex64 probe modes (``probe_model.probe``), corner-centred.
+        positions: (B, 2) float32 scan positions in pixels (``dset.scan_positions_px[batch]``).
+        propagators: (S-1, N, N) complex64 (``ptycho.propagators``), or None for one slice.
+        targets: (B, N, N) float32 fftshifted targets (``dset.targets[batch]``).
+        mask: (N, N) float32 detector mask (``dset.detector_mask``), default ones.
+        descan: (B, 2) float32 descan shifts or None (``dataset_models.py:1495-1498``).
+        num_gpts: total number of scan positions of the dataset.
+        mean_intensity: ``dset.mean_diffraction_intensity``.
+        loss_type: one of l2_amplitude, l1_amplitude, l2_intensity, l1_intensity, poisson.
+        return_intensities: also return the (B, N, N) fftshifted predicted intensities.
+
+    Returns:
+        loss (0-d float32 tensor), and intensities if requested.
+    """
+    if loss_type not in nat.LOSS_TYPES:
+        raise ValueError(f"Unknown loss type {loss_type}, should be 'l1' or 'l2'")
+    loss, intens = MultisliceLoss.apply(obj, probe, positions, propagators, targets, mask, descan, num_gpts,
+                                        mean_intensity, loss_type, return_intensities, torch.is_grad_enabled())
+    return (loss, intens) if return_intensities else loss
+
+
+def forward_intensities(
+    obj: torch.Tensor,
+    probe: torch.Tensor,
+    positions: torch.Tensor,
+    propagators: Optional[torch.Tensor],
+    descan: Optional[torch.Tensor] = None,
+) -> torch.Tensor:
+    """Predicted fftshifted intensities (B, N, N) of the forward model (no gradient)."""
+    B = positions.shape[0]
+    N = probe.shape[-1]
+    tgt = torch.zeros((B, N, N), dtype=torch.float32, device=obj.device)
+    with torch.no_grad():
+        _, intens = multislice_loss(obj, probe, positions, propagators, tgt, None, descan, num_gpts=B,
+                                    mean_intensity=1.0, return_intensities=True)
+    return intens
+
+
+def _fft2(x: torch.Tensor, inverse: bool) -> torch.Tensor:
+    if x.dtype != torch.complex64:
+        raise TypeError(f"fft2 needs complex64, got {x.dtype}")
+    if x.dim() < 2 or x.shape[-1] != x.shape[-2] or x.shape[-1] not in nat.SUPPORTED_ROI:
+        raise ValueError(f"fft2 needs (..., N, N) with N in {nat.SUPPORTED_ROI}, got {tuple(x.shape)}")
+    if x.device.type != "cuda":
+        raise ValueError(f"fft2 input must be on a GPU device, got {x.device}")
+    xc = x.contiguous()
+    out = torch.empty_like(xc)
+    n = x.shape[-1]
+    count = xc.numel() // (n * n)
+    rc = nat.load().qem_fft2_c64(n, count, 1 if inverse else 0, _ptr(xc), _ptr(out), _stream(x.device))
+    nat.check(rc, "qem_fft2_c64")
+    return out
+
+
+def fft2(x: torch.Tensor) -> torch.Tensor:
+    """torch.fft.fft2 (default norm) over the last two dims, N in {32, 64, 128}."""
+    return _fft2(x, False)
+
+
+def ifft2(x: torch.Tensor) -> torch.Tensor:
+    """torch.fft.ifft2 (default norm, 1/N^2) over the last two dims, N in {32, 64, 128}."""
+    return _fft2(x, True)

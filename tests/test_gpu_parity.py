@@ -1,0 +1,141 @@
+"""GPU parity of the HIP kernels (through the C ABI) against the FP64 oracle.
+
+Tolerances (relative L2 error against the float64 oracle evaluated on the same complex64 inputs):
+  FFT                          2e-6
+  intensities, loss            1e-5
+  object / probe / position /  1e-4
+  descan gradients
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import ptycho_oracle as po
+from quantem_amd import ptychography as qp
+from tests._cases import attach_targets, make_case, rel
+
+pytestmark = pytest.mark.gpu
+
+FWD_TOL = 1e-5
+GRAD_TOL = 1e-4
+
+
+def _t(x, dev):
+    return None if x is None else torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+
+
+def _run(case, dev, grads=True, pos_grad=True, desc_grad=True):
+    obj = _t(case["obj"], dev).requires_grad_(grads)
+    probe = _t(case["probe"], dev).requires_grad_(grads)
+    pos = _t(case["positions"], dev).requires_grad_(grads and pos_grad)
+    props = _t(case["propagators"], dev) if case["obj"].shape[0] > 1 else None
+    d = _t(case["descan"], dev)
+    if d is not None:
+        d.requires_grad_(grads and desc_grad)
+    loss, intens = qp.multislice_loss(obj, probe, pos, props, _t(case["targets"], dev), _t(case["mask"], dev), d,
+                                      num_gpts=case["num_gpts"], mean_intensity=case["mean_intensity"],
+                                      loss_type=case["loss_type"], return_intensities=True)
+    out = {"loss": loss.item(), "I": intens.cpu().numpy()}
+    if grads:
+        loss.backward()
+        out["obj"] = obj.grad.cpu().numpy()
+        out["probe"] = probe.grad.cpu().numpy()
+        if pos_grad:
+            out["positions"] = pos.grad.cpu().numpy()
+        if d is not None and desc_grad:
+            out["descan"] = d.grad.cpu().numpy()
+    return out
+
+
+def _oracle(case):
+    return po.adjoint(case["obj"], case["probe"], case["positions"],
+                      case["propagators"] if case["obj"].shape[0] > 1 else None, case["targets"], case["mask"],
+                      case["num_gpts"], case["mean_intensity"], case["loss_type"], descan=case["descan"])
+
+
+def _case_with_targets(**kw):
+    case = make_case(**kw)
+    I = po.forward(case["obj"], case["probe"], case["positions"], case["propagators"], case["descan"])["I"]
+    return attach_targets(case, I)
+
+
+@pytest.mark.parametrize("n", [32, 64, 128])
+def test_fft2_matches_numpy(cuda, n):
+    print(f"fft2 n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    x = (rng.normal(size=(5, n, n)) + 1j * rng.normal(size=(5, n, n))).astype(np.complex64)
+    xt = torch.from_numpy(x).to(cuda)
+    f = qp.fft2(xt).cpu().numpy()
+    assert rel(f, np.fft.fft2(x.astype(np.complex128))) < 2e-6
+    g = qp.ifft2(xt).cpu().numpy()
+    assert rel(g, np.fft.ifft2(x.astype(np.complex128))) < 2e-6
+
+
+@pytest.mark.parametrize(
+    "n,s,m,b,descan",
+    [(32, 1, 1, 5, False), (32, 3, 2, 6, True), (64, 1, 1, 7, False), (64, 4, 3, 5, False),
+     (128, 1, 1, 3, False), (128, 3, 2, 4, True), (128, 8, 4, 3, False)],
+)
+def test_forward_and_gradients_match_oracle(cuda, n, s, m, b, descan):
+    print(f"parity n={n} s={s} m={m} b={b} descan={descan}", flush=True)
+    case = _case_with_targets(N=n, S=s, M=m, B=b, seed=100 + n + s, descan=descan)
+    got = _run(case, cuda)
+    ref = _oracle(case)
+    assert rel(got["I"], ref["I"]) < FWD_TOL
+    assert abs(got["loss"] - ref["loss"]) / abs(ref["loss"]) < FWD_TOL
+    for k in ("obj", "probe", "positions") + (("descan",) if descan else ()):
+        assert rel(got[k], ref[k]) < GRAD_TOL, k
+
+
+@pytest.mark.parametrize("loss_type", ["l2_amplitude", "l1_amplitude", "l2_intensity", "l1_intensity", "poisson"])
+def test_loss_types(cuda, loss_type):
+    print(f"loss {loss_type}", flush=True)
+    case = _case_with_targets(N=32, S=2, M=2, B=6, seed=7, loss_type=loss_type)
+    got = _run(case, cuda)
+    ref = _oracle(case)
+    assert abs(got["loss"] - ref["loss"]) / abs(ref["loss"]) < FWD_TOL
+    for k in ("obj", "probe", "positions"):
+        assert rel(got[k], ref[k]) < GRAD_TOL, k
+
+
+def test_no_grad_forward_matches(cuda):
+    case = _case_with_targets(N=64, S=2, M=2, B=4, seed=3)
+    with torch.no_grad():
+        got = _run(case, cuda, grads=False)
+    ref = _oracle(case)
+    assert rel(got["I"], ref["I"]) < FWD_TOL
+
+
+def test_deterministic(cuda):
+    case = _case_with_targets(N=64, S=3, M=2, B=16, seed=11)
+    a = _run(case, cuda)
+    b = _run(case, cuda)
+    for k in ("I", "obj", "probe", "positions"):
+        assert np.array_equal(a[k], b[k]), k
+    assert a["loss"] == b["loss"]
+
+
+def test_object_equal_to_patch_size_wraps(cuda):
+    """Object as large as the patch (the reference test's 64x64 object with 64x64 patches)."""
+    case = _case_with_targets(N=64, S=1, M=2, B=5, H=64, W=64, seed=5)
+    got = _run(case, cuda)
+    ref = _oracle(case)
+    assert rel(got["I"], ref["I"]) < FWD_TOL
+    assert rel(got["obj"], ref["obj"]) < GRAD_TOL
+
+
+def test_argument_errors(cuda):
+    case = _case_with_targets(N=32, S=2, M=1, B=2, seed=1)
+    dev = cuda
+    args = dict(num_gpts=8, mean_intensity=1.0)
+    obj, probe, pos = _t(case["obj"], dev), _t(case["probe"], dev), _t(case["positions"], dev)
+    props, tg = _t(case["propagators"], dev), _t(case["targets"], dev)
+    with pytest.raises(TypeError):
+        qp.multislice_loss(obj.to(torch.complex128), probe, pos, props, tg, **args)
+    with pytest.raises(ValueError):
+        qp.multislice_loss(obj, probe, pos, props[:0], tg, **args)
+    with pytest.raises(ValueError):
+        qp.multislice_loss(obj, probe, pos, props, tg, loss_type="l3", **args)
+    with pytest.raises(ValueError):
+        qp.multislice_loss(obj, probe, pos, props, tg, num_gpts=8, mean_intensity=0.0)
