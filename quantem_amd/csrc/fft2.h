@@ -206,6 +206,8 @@ __device__ __forceinline__ cf twN(int j) {
   return kTw256[(j * (256 / N)) & 255];
 }
 
+
+
 template <int N>
 struct Fft2 {
   static constexpr int RC1 = Plan<N>::RC1, RC2 = Plan<N>::RC2;
@@ -225,6 +227,13 @@ struct Fft2 {
   static constexpr int LDS2 = (N - 1) + B3 * (RR3 - 1) + B2 * (RR2 - 1) + B1 * (RR1 - 1) + 1;
   static constexpr int LDS = LDS1 > LDS2 ? LDS1 : LDS2;  // complex64 slots of the exchange buffer
   static_assert(A1 >= RC2 && A2 >= RC2 + A1 * (RC1 - 1) && B3 >= N, "strides must not overlap");
+  // Twiddle tables kept in LDS after the exchange buffer (all accesses are base + immediate):
+  //   [c0][kc0] = W_N^(c0*kc0), rows padded to RC1+1 entries (bank-conflict free across c0)
+  //   [r3][k2]  = W_N^(RR1*r3*k2)
+  static constexpr int TWC = 0, TWC_ROW = RC1 + 1;
+  static constexpr int TWR = TWC + RC2 * TWC_ROW;
+  static constexpr int TWL = TWR + RR3 * RR2;
+  static constexpr int LDS_TOTAL = LDS + TWL;  // complex64 slots a kernel reserves: exchange + twiddles
 
   // P1 role
   int rp, c0;
@@ -233,14 +242,14 @@ struct Fft2 {
   // P3 role
   int kc, q;
   int p1base;          // rp*N + c0
+  const cf* twl;       // LDS twiddle tables
+  int twc_base, twr_base;
+  cf tw_rp;            // W_N^rp
   int e1p1, e1p2;      // exchange-1 base addresses of this thread in layouts P1 and P2
   int e2p2;            // exchange-2 base address in layout P2
   int e2p3[H3];        // exchange-2 base addresses in layout P3 (per register group)
   int p3k12[H3];       // k1 + RR1*k2 of each pass-3 register group
   int p3base[H3];      // p3k12*N + kc
-  cf tw_rp;        // W_N^rp
-  cf tw_c[RC1];    // W_N^(c0*kc0)
-  cf tw_r3[RR2];   // W_N^(RR1*r3*k2)
 
   __device__ __forceinline__ explicit Fft2(int tid) {
     c0 = tid % RC2;
@@ -257,12 +266,29 @@ struct Fft2 {
       p3base[h.value] = p3k12[h.value] * N + kc;
       e2p3[h.value] = kc + B2 * (j / RR1) + B1 * (j % RR1);
     });
+    tw_rp = twN<N>(rp);
+    twc_base = TWC + c0 * TWC_ROW;
+    twr_base = TWR + r3b * RR2;
     e1p1 = c0 + A2 * rp;
     e1p2 = A1 * kc0b + A2 * r3b + A3 * k1b;
     e2p2 = kc0b + B3 * r3b + B1 * k1b;
-    tw_rp = twN<N>(rp);
-    sfor<0, RC1>([&](auto i) { tw_c[i.value] = twN<N>(c0 * i.value); });
-    sfor<0, RR2>([&](auto i) { tw_r3[i.value] = twN<N>(RR1 * r3b * i.value); });
+  }
+
+  // Fills the LDS twiddle tables at lds + LDS; the caller must __syncthreads() before fwd()/inv().
+  __device__ __forceinline__ void init_twiddles(cf* lds, int tid) {
+    cf* t = lds + LDS;
+    for (int i = tid; i < TWL; i += T) {
+      int j;
+      if (i < TWR) {
+        const int row = i / TWC_ROW, col = i - row * TWC_ROW;
+        j = row * (col < RC1 ? col : 0);
+      } else {
+        const int r = (i - TWR) / RR2, k2 = (i - TWR) % RR2;
+        j = RR1 * r * k2;
+      }
+      t[i] = twN<N>(j);
+    }
+    twl = t;
   }
 
   // Natural coordinates of register e in layout P1 (real space).
@@ -282,7 +308,7 @@ struct Fft2 {
     sfor<0, RC1>([&](auto c1) { v[RC1 + c1.value] = cmul(v[RC1 + c1.value], tw_rp); });
     sfor<0, RR1>([&](auto k1) { Dft<RC1, false>::template run<1>(&v[k1.value * RC1]); });
     sfor<1, RC1>([&](auto kc0) {
-      sfor<0, RR1>([&](auto k1) { v[k1.value * RC1 + kc0.value] = cmul(v[k1.value * RC1 + kc0.value], tw_c[kc0.value]); });
+      sfor<0, RR1>([&](auto k1) { v[k1.value * RC1 + kc0.value] = cmul(v[k1.value * RC1 + kc0.value], twl[twc_base + kc0.value]); });
     });
     sfor<0, RR1>([&](auto k1) {
       sfor<0, RC1>([&](auto kc0) { lds[e1p1 + A3 * k1.value + A1 * kc0.value] = v[k1.value * RC1 + kc0.value]; });
@@ -298,7 +324,7 @@ struct Fft2 {
     sfor<0, RR2>([&](auto r2) { Dft<RC2, false>::template run<1>(&v[r2.value * RC2]); });
     sfor<0, RC2>([&](auto kc1) { Dft<RR2, false>::template run<RC2>(&v[kc1.value]); });
     sfor<1, RR2>([&](auto k2) {
-      sfor<0, RC2>([&](auto kc1) { v[k2.value * RC2 + kc1.value] = cmul(v[k2.value * RC2 + kc1.value], tw_r3[k2.value]); });
+      sfor<0, RC2>([&](auto kc1) { v[k2.value * RC2 + kc1.value] = cmul(v[k2.value * RC2 + kc1.value], twl[twr_base + k2.value]); });
     });
     sfor<0, RR2>([&](auto k2) {
       sfor<0, RC2>([&](auto kc1) {
@@ -327,7 +353,7 @@ struct Fft2 {
     });
     __syncthreads();
     sfor<1, RR2>([&](auto k2) {
-      sfor<0, RC2>([&](auto kc1) { v[k2.value * RC2 + kc1.value] = cmulc(v[k2.value * RC2 + kc1.value], tw_r3[k2.value]); });
+      sfor<0, RC2>([&](auto kc1) { v[k2.value * RC2 + kc1.value] = cmulc(v[k2.value * RC2 + kc1.value], twl[twr_base + k2.value]); });
     });
     sfor<0, RC2>([&](auto kc1) { Dft<RR2, true>::template run<RC2>(&v[kc1.value]); });
     sfor<0, RR2>([&](auto r2) { Dft<RC2, true>::template run<1>(&v[r2.value * RC2]); });
@@ -341,7 +367,7 @@ struct Fft2 {
       sfor<0, RC1>([&](auto kc0) { v[k1.value * RC1 + kc0.value] = lds[e1p1 + A3 * k1.value + A1 * kc0.value]; });
     });
     sfor<1, RC1>([&](auto kc0) {
-      sfor<0, RR1>([&](auto k1) { v[k1.value * RC1 + kc0.value] = cmulc(v[k1.value * RC1 + kc0.value], tw_c[kc0.value]); });
+      sfor<0, RR1>([&](auto k1) { v[k1.value * RC1 + kc0.value] = cmulc(v[k1.value * RC1 + kc0.value], twl[twc_base + kc0.value]); });
     });
     sfor<0, RR1>([&](auto k1) { Dft<RC1, true>::template run<1>(&v[k1.value * RC1]); });
     sfor<0, RC1>([&](auto c1) { v[RC1 + c1.value] = cmulc(v[RC1 + c1.value], tw_rp); });

@@ -90,6 +90,11 @@ __device__ __forceinline__ void origin_and_shift(const float* pos, int b, int& r
   dc = pc - rc;
 }
 
+// Object element at a byte offset from the slice base.
+__device__ __forceinline__ cf obj_at(const cf* O, unsigned byte_off) {
+  return *(const cf*)((const char*)O + byte_off);
+}
+
 // Block-wide sum of one double per thread; result valid in thread 0.
 template <int T>
 __device__ __forceinline__ double block_sum(double x, double* red) {
@@ -109,9 +114,11 @@ __device__ __forceinline__ double block_sum(double x, double* red) {
 // Probe spectra: pk[m] = FFT2(probe[m]).
 template <int N>
 __global__ __launch_bounds__(Fft2<N>::T) void k_probe_spectrum(Params p) {
-  __shared__ cf lds[Fft2<N>::LDS];
+  __shared__ cf lds[Fft2<N>::LDS_TOTAL];
   using F = Fft2<N>;
   F f(threadIdx.x);
+  f.init_twiddles(lds, threadIdx.x);
+  __syncthreads();
   const int m = blockIdx.x;
   cf v[F::PTS];
   const cf* src = p.probe + (size_t)m * N * N;
@@ -128,10 +135,12 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_probe_spectrum(Params p) {
 // for s: x_s = O_s[patch] * psi_s; psi_{s+1} = ifft2(fft2(x_s) * prop_s); X = fft2_ortho(x_{S-1}).
 template <int N, bool DESCAN>
 __global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
-  __shared__ cf lds[Fft2<N>::LDS];
+  __shared__ cf lds[Fft2<N>::LDS_TOTAL];
   using F = Fft2<N>;
   constexpr int PTS = F::PTS;
   F f(threadIdx.x);
+  f.init_twiddles(lds, threadIdx.x);
+  __syncthreads();
   const int item = xcd_item(blockIdx.x, gridDim.x);
   const int b = item / p.M, m = item - b * p.M;
   int r0, c0;
@@ -154,16 +163,17 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
   f.inv(v, lds);
 
   // object rows / cols of this thread's elements (periodic wrap, fftfreq offsets)
-  int orow[F::RR1], ocol[F::RC1];
+  // byte offsets (32-bit, from a uniform slice base) of this thread's object rows / cols
+  unsigned orow[F::RR1], ocol[F::RC1];
 #pragma unroll
   for (int i = 0; i < F::RR1; ++i) {
     const int r = i * F::NRP + f.rp;
-    orow[i] = pmod(r0 + (r < N / 2 ? r : r - N), p.H) * p.W;
+    orow[i] = (unsigned)(pmod(r0 + (r < N / 2 ? r : r - N), p.H) * p.W) * 8u;
   }
 #pragma unroll
   for (int i = 0; i < F::RC1; ++i) {
     const int c = i * F::RC2 + f.c0;
-    ocol[i] = pmod(c0 + (c < N / 2 ? c : c - N), p.W);
+    ocol[i] = (unsigned)pmod(c0 + (c < N / 2 ? c : c - N), p.W) * 8u;
   }
 
   const bool store = (p.flags & QEM_FLAG_STORE_WAVES) != 0;
@@ -177,7 +187,7 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
       for (int e = 0; e < PTS; ++e) dst[f.p1_off(e)] = v[e];
     }
 #pragma unroll
-    for (int e = 0; e < PTS; ++e) v[e] = cmul(O[orow[e / F::RC1] + ocol[e % F::RC1]], v[e]);
+    for (int e = 0; e < PTS; ++e) v[e] = cmul(obj_at(O, orow[e / F::RC1] + ocol[e % F::RC1]), v[e]);
     if (s + 1 < p.S) {
       f.fwd(v, lds);
       const cf* pr = p.prop + (size_t)s * N * N;
@@ -262,11 +272,13 @@ __global__ __launch_bounds__(256) void k_loss_finalize(Params p, int nparts) {
 // the position gradient reduces Im(conj(G) P_m ramp_b) against the frequencies.
 template <int N, bool DESCAN>
 __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
-  __shared__ cf lds[Fft2<N>::LDS];
+  __shared__ cf lds[Fft2<N>::LDS_TOTAL];
   __shared__ double red[Fft2<N>::T / 64];
   using F = Fft2<N>;
   constexpr int PTS = F::PTS;
   F f(threadIdx.x);
+  f.init_twiddles(lds, threadIdx.x);
+  __syncthreads();
   const int item = xcd_item(blockIdx.x, gridDim.x);
   const int b = item / p.M, m = item - b * p.M;
   int r0, c0;
@@ -289,16 +301,17 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
   }
   f.inv(v, lds);
 
-  int orow[F::RR1], ocol[F::RC1];
+  // byte offsets (32-bit, from a uniform slice base) of this thread's object rows / cols
+  unsigned orow[F::RR1], ocol[F::RC1];
 #pragma unroll
   for (int i = 0; i < F::RR1; ++i) {
     const int r = i * F::NRP + f.rp;
-    orow[i] = pmod(r0 + (r < N / 2 ? r : r - N), p.H) * p.W;
+    orow[i] = (unsigned)(pmod(r0 + (r < N / 2 ? r : r - N), p.H) * p.W) * 8u;
   }
 #pragma unroll
   for (int i = 0; i < F::RC1; ++i) {
     const int c = i * F::RC2 + f.c0;
-    ocol[i] = pmod(c0 + (c < N / 2 ? c : c - N), p.W);
+    ocol[i] = (unsigned)pmod(c0 + (c < N / 2 ? c : c - N), p.W) * 8u;
   }
 
   const size_t HW = (size_t)p.H * p.W;
@@ -316,7 +329,7 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
       const float fr = fftfreq1<N>(row), fc = fftfreq1<N>(col);
       const cf ramp = cmul(expm2pi(fr * ddr), expm2pi(fc * ddc));
       if (want_desc) {
-        const cf xo = cmul(cmul(O[orow[e / F::RC1] + ocol[e % F::RC1]], ps[f.p1_off(e)]), ramp);
+        const cf xo = cmul(cmul(obj_at(O, orow[e / F::RC1] + ocol[e % F::RC1]), ps[f.p1_off(e)]), ramp);
         const float im = cconjmul(v[e], xo).y;
         ar += (double)(fr * im);
         ac += (double)(fc * im);
@@ -332,25 +345,39 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
       }
     }
   }
-  for (int s = p.S - 1; s >= 0; --s) {
+  // one slice back: object-gradient patch conj(psi_s) g_x (written over psi_s, all loads before
+  // stores), then g_psi = conj(O_s) g_x
+  auto slice_step = [&](int s) {
     const cf* O = p.obj + (size_t)s * HW;
     cf* ps = p.psi + ((size_t)item * p.S + s) * N * N;
+    // chunks of 8 elements: the psi stores of one chunk keep the next chunk's loads from being
+    // hoisted, which bounds the registers in flight
 #pragma unroll
-    for (int e = 0; e < PTS; ++e) {
-      const int a = f.p1_off(e);
-      const cf psi = ps[a];
-      const cf o = O[orow[e / F::RC1] + ocol[e % F::RC1]];
-      ps[a] = cconjmul(psi, v[e]);
-      v[e] = cconjmul(o, v[e]);
+    for (int e0 = 0; e0 < PTS; e0 += 8) {
+      cf t[8], o[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        t[i] = ps[f.p1_off(e0 + i)];
+        o[i] = obj_at(O, orow[(e0 + i) / F::RC1] + ocol[(e0 + i) % F::RC1]);
+      }
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        ps[f.p1_off(e0 + i)] = cconjmul(t[i], v[e0 + i]);
+        v[e0 + i] = cconjmul(o[i], v[e0 + i]);
+      }
     }
+  };
+  for (int s = p.S - 1; s >= 1; --s) {
+    slice_step(s);
+    // g_x <- ifft2(fft2(g_psi) conj(prop_{s-1})), the adjoint of the forward propagation
     f.fwd(v, lds);
-    if (s > 0) {
-      const cf* pr = p.prop + (size_t)(s - 1) * N * N;
+    const cf* __restrict__ pr = p.prop + (size_t)(s - 1) * N * N;
 #pragma unroll
-      for (int e = 0; e < PTS; ++e) v[e] = cscale(cmulc(v[e], pr[f.p3_off(e)]), inv_n2);
-      f.inv(v, lds);
-    }
+    for (int e = 0; e < PTS; ++e) v[e] = cscale(cmulc(v[e], pr[f.p3_off(e)]), inv_n2);
+    f.inv(v, lds);
   }
+  slice_step(0);
+  f.fwd(v, lds);
   // v = G = fft2(g_psi_0) in P3.
   const cf rc = expm2pi(fftfreq1<N>(f.kc) * dc);
   const cf* pk = p.pk + (size_t)m * N * N;
@@ -364,11 +391,9 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
     const cf ramp = cmul(expm2pi(fr * dr), rc);
     const int k = f.p3_off(e);
     Xs[k] = cmulc(v[e], ramp);
-    if (want_pos) {
-      const float im = cconjmul(v[e], cmul(pk[k], ramp)).y;
-      ar += (double)(fr * im);
-      ac += (double)(fc * im);
-    }
+    const float im = cconjmul(v[e], cmul(pk[k], ramp)).y;  // cheap; kept branch-free
+    ar += (double)(fr * im);
+    ac += (double)(fc * im);
   }
   if (want_pos) {
     const double tr = block_sum<F::T>(ar, red);
@@ -382,66 +407,120 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
 
 // ---------------------------------------------------------------------------------------------
 // Deterministic object-gradient scatter-add: each block owns a 32x32 tile of one slice and sums,
-// in increasing (b, m) order, the per-mode patch gradients that land on it.
+// in increasing (b, m) order, the per-mode patch gradients that land on it.  Patch row index of
+// object row y for a patch with origin r0: u = (y - r0 + N/2) mod H (valid if u < N), stored row
+// (u + N/2) mod N (fftfreq order).  The block first tabulates, for 256 patterns at a time, the
+// tile origin in patch coordinates (or -1 when the patch misses the tile), then walks the table.
 template <int N>
 __global__ __launch_bounds__(256) void k_obj_reduce(Params p) {
+  __shared__ int su0[256], sv0[256];
   const int s = blockIdx.z;
-  const int x = blockIdx.x * 32 + (threadIdx.x & 31);
-  const int ybase = blockIdx.y * 32 + (threadIdx.x >> 5);
+  const int X0 = blockIdx.x * 32, Y0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int x = X0 + tx;
   cf acc[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) acc[j] = mk(0.0f, 0.0f);
-  const bool xin = x < p.W;
-  for (int b = 0; b < p.B; ++b) {
-    const float pr = p.pos[2 * b], pc = p.pos[2 * b + 1];
-    const int r0 = (int)rintf(pr), c0 = (int)rintf(pc);
-    const int ux = pmod(x - c0 + N / 2, p.W);
-    if (!xin || ux >= N) continue;
-    const int col = (ux + N / 2) & (N - 1);
+  const size_t slab = (size_t)p.S * N * N;  // stride between modes of one pattern
+  for (int b0 = 0; b0 < p.B; b0 += 256) {
+    const int nb = min(256, p.B - b0);
+    if ((int)threadIdx.x < nb) {
+      const int b = b0 + threadIdx.x;
+      const int r0 = (int)rintf(p.pos[2 * b]), c0 = (int)rintf(p.pos[2 * b + 1]);
+      const int u0 = pmod(Y0 - r0 + N / 2, p.H), v0 = pmod(X0 - c0 + N / 2, p.W);
+      const bool hit = (u0 < N || u0 + 31 >= p.H) && (v0 < N || v0 + 31 >= p.W);
+      su0[threadIdx.x] = hit ? u0 : -1;
+      sv0[threadIdx.x] = v0;
+    }
+    __syncthreads();
+    for (int i = 0; i < nb; ++i) {
+      const int u0 = su0[i];
+      if (u0 < 0) continue;
+      int ux = sv0[i] + tx;
+      if (ux >= p.W) ux -= p.W;
+      const bool xok = x < p.W && ux < N;
+      const int col = (ux + N / 2) & (N - 1);
+      const cf* g = p.psi + ((size_t)(b0 + i) * p.M * p.S + s) * N * N + col;
+      const cf* rowp[4];
+      bool ok[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int y = ybase + 8 * j;
-      const int uy = pmod(y - r0 + N / 2, p.H);
-      if (y < p.H && uy < N) {
-        const int row = (uy + N / 2) & (N - 1);
-        const cf* g = p.psi + (((size_t)b * p.M) * p.S + s) * N * N + row * N + col;
-        for (int m = 0; m < p.M; ++m) {
-          const cf t = g[(size_t)m * p.S * N * N];
-          acc[j].x += t.x;
-          acc[j].y += t.y;
+      for (int j = 0; j < 4; ++j) {
+        const int dy = ty + 8 * j;
+        int uy = u0 + dy;
+        if (uy >= p.H) uy -= p.H;
+        ok[j] = xok && (Y0 + dy < p.H) && uy < N;
+        rowp[j] = g + ((uy + N / 2) & (N - 1)) * N;
+      }
+      // all loads of a pattern are issued before any add (masked lanes read a valid dummy address)
+      for (int m0 = 0; m0 < p.M; m0 += 4) {
+        cf t[4][4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+#pragma unroll
+          for (int mm = 0; mm < 4; ++mm) {
+            const bool use = ok[j] && (m0 + mm < p.M);
+            t[j][mm] = *(use ? rowp[j] + (size_t)(m0 + mm) * slab : (const cf*)p.psi);
+            if (!use) t[j][mm] = mk(0.0f, 0.0f);
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+#pragma unroll
+          for (int mm = 0; mm < 4; ++mm) {
+            acc[j].x += t[j][mm].x;
+            acc[j].y += t[j][mm].y;
+          }
         }
       }
     }
+    __syncthreads();
   }
-  if (!xin) return;
+  if (x >= p.W) return;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    const int y = ybase + 8 * j;
+    const int y = Y0 + ty + 8 * j;
     if (y < p.H) p.grad_obj[(size_t)s * p.H * p.W + (size_t)y * p.W + x] = acc[j];
   }
 }
 
-// gsum[m][k] = sum_b spectra[b][m][k], fixed order.
+// gsum[m][k] = sum_b spectra[b][m][k] in a fixed order: 64 frequencies per block, 4 lanes per
+// frequency each summing every 4th pattern, combined in lane order through LDS.
 template <int N>
 __global__ __launch_bounds__(256) void k_probe_reduce(Params p) {
-  const int idx = blockIdx.x * 256 + threadIdx.x;  // over M*N*N
+  __shared__ cf part[4][64];
+  const int kk = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int idx = blockIdx.x * 64 + kk;  // over M*N*N (multiple of 64)
   const int m = idx / (N * N);
   const int k = idx - m * N * N;
   cf acc = mk(0.0f, 0.0f);
-  for (int b = 0; b < p.B; ++b) {
-    const cf t = p.X[((size_t)b * p.M + m) * N * N + k];
+  const cf* src = p.X + (size_t)m * N * N + k;
+  const size_t stride = (size_t)p.M * N * N;
+#pragma unroll 8
+  for (int b = g; b < p.B; b += 4) {
+    const cf t = src[(size_t)b * stride];
     acc.x += t.x;
     acc.y += t.y;
   }
-  p.gsum[(size_t)m * N * N + k] = acc;
+  part[g][kk] = acc;
+  __syncthreads();
+  if (g == 0) {
+    cf t = part[0][kk];
+    for (int i = 1; i < 4; ++i) {
+      t.x += part[i][kk].x;
+      t.y += part[i][kk].y;
+    }
+    p.gsum[(size_t)idx] = t;
+  }
 }
 
 // grad_probe[m] = ifft2(gsum[m]) (1/N^2 included).
 template <int N>
 __global__ __launch_bounds__(Fft2<N>::T) void k_probe_grad(Params p) {
-  __shared__ cf lds[Fft2<N>::LDS];
+  __shared__ cf lds[Fft2<N>::LDS_TOTAL];
   using F = Fft2<N>;
   F f(threadIdx.x);
+  f.init_twiddles(lds, threadIdx.x);
+  __syncthreads();
   const int m = blockIdx.x;
   cf v[F::PTS];
   const cf* src = p.gsum + (size_t)m * N * N;
@@ -467,9 +546,11 @@ __global__ __launch_bounds__(256) void k_pair_reduce(const float* part, float* o
 // Batched 2-D FFT utility (torch default norms).
 template <int N>
 __global__ __launch_bounds__(Fft2<N>::T) void k_fft2(const cf* in, cf* out, int inverse) {
-  __shared__ cf lds[Fft2<N>::LDS];
+  __shared__ cf lds[Fft2<N>::LDS_TOTAL];
   using F = Fft2<N>;
   F f(threadIdx.x);
+  f.init_twiddles(lds, threadIdx.x);
+  __syncthreads();
   const size_t base = (size_t)blockIdx.x * N * N;
   cf v[F::PTS];
   if (!inverse) {
@@ -568,8 +649,9 @@ int check_desc(const qem_ptycho_desc* d) {
   if (d->obj_rows < d->roi || d->obj_cols < d->roi)
     return fail(QEM_ERR_ARG, "object (%d, %d) must be at least roi %d in each dimension", d->obj_rows, d->obj_cols,
                 d->roi);
-  if ((size_t)d->num_slices * d->obj_rows * d->obj_cols >= ((size_t)1 << 31))
-    return fail(QEM_ERR_ARG, "object has too many elements");
+  if ((size_t)d->num_slices * d->obj_rows * d->obj_cols >= ((size_t)1 << 31) ||
+      (size_t)d->obj_rows * d->obj_cols > ((size_t)1 << 28))
+    return fail(QEM_ERR_ARG, "object has too many elements (at most 2^28 per slice, 2^31 in total)");
   if (d->num_gpts < 1) return fail(QEM_ERR_ARG, "num_gpts must be >= 1 (got %d)", d->num_gpts);
   if (d->loss_type < QEM_LOSS_L2_AMPLITUDE || d->loss_type > QEM_LOSS_POISSON)
     return fail(QEM_ERR_ARG, "unknown loss_type %d", d->loss_type);
@@ -645,7 +727,7 @@ int launch_adjoint(const qem::Params& p, hipStream_t st) {
   dim3 g((p.W + 31) / 32, (p.H + 31) / 32, p.S);
   hipLaunchKernelGGL(qem::k_obj_reduce<N>, g, dim3(256), 0, st, p);
   if (int rc = check_launch("obj_reduce")) return rc;
-  hipLaunchKernelGGL(qem::k_probe_reduce<N>, dim3(p.M * N * N / 256), dim3(256), 0, st, p);
+  hipLaunchKernelGGL(qem::k_probe_reduce<N>, dim3(p.M * N * N / 64), dim3(256), 0, st, p);
   if (int rc = check_launch("probe_reduce")) return rc;
   hipLaunchKernelGGL(qem::k_probe_grad<N>, dim3(p.M), dim3(F::T), 0, st, p);
   if (int rc = check_launch("probe_grad")) return rc;
