@@ -1,0 +1,158 @@
+"""Reconstruction loop around the fused kernels, mirroring quantem's ``Ptychography.reconstruct``.
+
+The loop body is the reference's (``ptychography.py:208-245``): a fresh random permutation of the scan
+positions each epoch (``SimpleBatcher``, ``ptycho_utils.py:16-117``), hard constraints on the object and
+probe on the autograd graph (``object_models.py:289-339``, ``probe_models.py:452-525``), the batch loss
+and its gradient (here one call of :func:`quantem_amd.ptychography.multislice_loss`), the object
+gradient rescale by ``upsample_factor**2 / 2`` (``ptychography.py:330-332``) and an Adam step with the
+reference's default learning rates (object 5e-3, probe 1e-3; ``object_models.py:44-48``,
+``probe_models.py:52-54``).  The object starts flat (``ObjectPixelated.from_uniform``,
+``object_models.py:549-571, 653-657``) unless an initial object is given (the warm start).
+"""
+
+from __future__ import annotations
+
+import time
+from math import ceil
+from typing import Callable, Optional
+
+import numpy as np
+import torch
+
+from .ptychography import multislice_loss
+
+
+class SimpleBatcher:
+    """Random mini-batches of scan indices, a new permutation every epoch (ptycho_utils.py:16-117)."""
+
+    def __init__(self, num: int, batch_size: Optional[int], shuffle: bool = True, rng=None):
+        self.indices = np.arange(num)
+        self.batch_size = batch_size if batch_size is not None else num
+        self.shuffle = shuffle
+        if rng is None or isinstance(rng, (int, np.integer)):
+            rng = np.random.default_rng(rng)
+        if not isinstance(rng, np.random.Generator):
+            raise TypeError(f"rng should be a np.random.Generator or a seed, got {type(rng)}")
+        self.rng = rng
+
+    def __iter__(self):
+        order = self.rng.permutation(self.indices) if self.shuffle else self.indices
+        for i in range(0, len(order), self.batch_size):
+            yield order[i:i + self.batch_size]
+
+    def __len__(self) -> int:
+        return int(ceil(len(self.indices) / self.batch_size))
+
+
+def object_hard_constraints(obj: torch.Tensor, obj_type: str = "complex") -> torch.Tensor:
+    """ObjectConstraints.apply_hard_constraints for the default constraint set (object_models.py:289-339)."""
+    if obj_type in ("complex", "pure_phase"):
+        amp = torch.clamp(torch.abs(obj), 0.0, 1.0) if obj_type == "complex" else 1.0
+        ang = obj.angle()
+        phase = ang - ang.mean()
+        return amp * torch.exp(1.0j * phase)
+    if obj_type == "potential":
+        return torch.exp(1.0j * torch.clamp(obj, min=0.0))
+    raise ValueError(f"unknown obj_type {obj_type}")
+
+
+def orthogonalize_probe(probe: torch.Tensor) -> torch.Tensor:
+    """Gram-Schmidt keeping each mode's norm, modes sorted by power (probe_models.py:484-525)."""
+    norms = torch.sqrt(torch.sum(probe.real.square() + probe.imag.square(), dim=(-2, -1), keepdim=True))
+    out = []
+    for i in range(probe.shape[0]):
+        p = probe[i]
+        for q in out:
+            p = p - torch.sum(q.conj() * p) * q
+        n = torch.sqrt(torch.sum(p.real.square() + p.imag.square())).clamp_min(1e-12)
+        out.append(p / n)
+    out = torch.stack(out) * norms.view(-1, 1, 1)
+    order = torch.argsort(torch.sum(torch.abs(out).square(), dim=(-2, -1)), descending=True)
+    return torch.complex(out.real[order], out.imag[order])
+
+
+def projected_phase(obj: torch.Tensor) -> torch.Tensor:
+    """Sum over slices of the transmission phase."""
+    return torch.sum(torch.angle(obj), dim=0)
+
+
+def phase_nmse(est: np.ndarray, true: np.ndarray, crop: Optional[tuple] = None) -> float:
+    """||(est - true) - mean(est - true)||^2 / ||true - mean(true)||^2 over the crop (constant phase removed)."""
+    if crop is not None:
+        (r0, r1), (c0, c1) = crop
+        est = est[r0:r1, c0:c1]
+        true = true[r0:r1, c0:c1]
+    d = est - true
+    d = d - d.mean()
+    t = true - true.mean()
+    return float(np.sum(d * d) / max(np.sum(t * t), 1e-300))
+
+
+class Reconstruction:
+    """Gradient-descent multislice reconstruction on one GPU (the reference's autograd=True mode)."""
+
+    def __init__(self, positions, targets, propagators, probe, obj_shape, mean_intensity: float,
+                 mask=None, obj_type: str = "complex", orthogonalize: bool = True, upsample_factor: float = 1.0,
+                 obj_init: Optional[torch.Tensor] = None, device="cuda"):
+        dev = torch.device(device)
+        self.device = dev
+        self.positions = torch.as_tensor(positions, dtype=torch.float32, device=dev).contiguous()
+        self.targets = torch.as_tensor(targets, dtype=torch.float32, device=dev).contiguous()
+        S = int(obj_shape[0])
+        self.propagators = (torch.as_tensor(propagators, dtype=torch.complex64, device=dev).contiguous()
+                            if S > 1 else None)
+        self.mask = None if mask is None else torch.as_tensor(mask, dtype=torch.float32, device=dev)
+        self.mean_intensity = float(mean_intensity)
+        self.num_gpts = int(self.positions.shape[0])
+        self.obj_type = obj_type
+        self.orthogonalize = orthogonalize
+        self.upsample_factor = float(upsample_factor)
+        if obj_init is None:
+            obj0 = torch.ones(tuple(int(x) for x in obj_shape), dtype=torch.complex64, device=dev)
+        else:
+            obj0 = torch.as_tensor(obj_init, dtype=torch.complex64, device=dev).clone()
+        self.obj = torch.nn.Parameter(obj0)
+        self.probe = torch.nn.Parameter(torch.as_tensor(probe, dtype=torch.complex64, device=dev).clone())
+        self.history: list = []
+        self.times: list = []
+
+    @property
+    def obj_constrained(self) -> torch.Tensor:
+        return object_hard_constraints(self.obj, self.obj_type)
+
+    @property
+    def probe_constrained(self) -> torch.Tensor:
+        return orthogonalize_probe(self.probe) if (self.orthogonalize and self.probe.shape[0] > 1) else self.probe
+
+    def reconstruct(self, num_iters: int, batch_size: int, obj_lr: float = 5e-3, probe_lr: float = 1e-3,
+                    learn_probe: bool = True, loss_type: str = "l2_amplitude", seed: int = 0,
+                    callback: Optional[Callable[[int, float, "Reconstruction"], None]] = None) -> list:
+        """Runs `num_iters` epochs; returns the per-epoch mean batch loss."""
+        groups = [{"params": [self.obj], "lr": obj_lr}]
+        if learn_probe:
+            groups.append({"params": [self.probe], "lr": probe_lr})
+        else:
+            self.probe.requires_grad_(False)
+        opt = torch.optim.Adam(groups)
+        batcher = SimpleBatcher(self.num_gpts, batch_size, rng=seed)
+        scale = self.upsample_factor ** 2 / 2
+        t0 = time.perf_counter()
+        for epoch in range(num_iters):
+            total = torch.zeros((), dtype=torch.float32, device=self.device)
+            for idx in batcher:
+                it = torch.from_numpy(idx).to(self.device)
+                opt.zero_grad(set_to_none=False)
+                loss = multislice_loss(self.obj_constrained, self.probe_constrained, self.positions[it],
+                                       self.propagators, self.targets[it], self.mask, None,
+                                       num_gpts=self.num_gpts, mean_intensity=self.mean_intensity,
+                                       loss_type=loss_type)
+                loss.backward()
+                self.obj.grad.mul_(scale)
+                opt.step()
+                total = total + loss.detach()
+            mean_loss = float(total.item()) / len(batcher)  # one host sync per epoch
+            self.history.append(mean_loss)
+            self.times.append(time.perf_counter() - t0)
+            if callback is not None:
+                callback(epoch, mean_loss, self)
+        return self.history

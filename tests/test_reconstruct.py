@@ -1,0 +1,75 @@
+"""Reconstruction loop mirror: host logic on the CPU, a short reconstruction on the GPU."""
+
+import numpy as np
+import pytest
+import torch
+
+from quantem_amd import reconstruct as rc
+from quantem_amd import synth
+
+
+def test_batcher_covers_every_index_each_epoch():
+    b = rc.SimpleBatcher(103, 10, rng=3)
+    assert len(b) == 11
+    for _ in range(3):
+        seen = np.concatenate(list(b))
+        assert sorted(seen.tolist()) == list(range(103))
+    b2 = rc.SimpleBatcher(10, None, shuffle=False)
+    assert [x.tolist() for x in b2] == [list(range(10))]
+    with pytest.raises(TypeError):
+        rc.SimpleBatcher(5, 2, rng="x")
+
+
+def test_object_constraints():
+    obj = torch.ones((2, 5, 5), dtype=torch.complex64)
+    assert torch.allclose(rc.object_hard_constraints(obj), obj)
+    big = 2.0 * torch.exp(1j * torch.full((1, 4, 4), 0.3))
+    out = rc.object_hard_constraints(big.to(torch.complex64))
+    assert torch.allclose(out.abs(), torch.ones(1, 4, 4), atol=1e-6)  # amplitude clamped to 1
+    assert abs(float(out.angle().mean())) < 1e-6  # mean phase removed
+    pp = rc.object_hard_constraints(big.to(torch.complex64), "pure_phase")
+    assert torch.allclose(pp.abs(), torch.ones(1, 4, 4), atol=1e-6)
+
+
+def test_orthogonalize_probe():
+    g = torch.Generator().manual_seed(0)
+    p = torch.randn((3, 16, 16), dtype=torch.complex64, generator=g)
+    q = rc.orthogonalize_probe(p)
+    gram = torch.einsum("aij,bij->ab", q.conj(), q)
+    off = gram - torch.diag(torch.diagonal(gram))
+    assert off.abs().max() < 1e-3 * gram.abs().max()
+    pw = torch.sum(q.abs() ** 2, dim=(-2, -1))
+    assert torch.all(pw[:-1] >= pw[1:])
+    assert torch.allclose(torch.sort(pw).values, torch.sort(torch.sum(p.abs() ** 2, dim=(-2, -1))).values, rtol=1e-4)
+
+
+def test_phase_nmse():
+    rng = np.random.default_rng(0)
+    t = rng.normal(size=(20, 20))
+    assert rc.phase_nmse(t + 3.0, t) < 1e-20
+    assert rc.phase_nmse(np.zeros_like(t), t) == pytest.approx(1.0)
+    assert rc.phase_nmse(t, t, crop=((2, 10), (3, 9))) == 0.0
+
+
+@pytest.mark.gpu
+def test_reconstruction_reduces_loss_and_error(cuda):
+    """Single-slice, single-mode problem (a reduced version of the CPU-runnable reference config)."""
+    spec = synth.ProblemSpec(name="single-slice-test", num_slices=1, obj_shape=(128, 128), roi=64, num_modes=1,
+                             mode_weights=(1.0,), scan=(12, 12), scan_step=4.0, jitter=0.5, dose=1e4, n_atoms=60,
+                             batch=32)
+    prob = synth.make_problem(spec, seed=1)
+    from quantem_amd.ptychography import forward_intensities
+
+    dev = cuda
+    obj = torch.from_numpy(prob["obj"]).to(dev)
+    probe = torch.from_numpy(prob["probe"]).to(dev)
+    pos = torch.from_numpy(prob["positions"]).to(dev)
+    I = forward_intensities(obj, probe, pos, None).cpu().numpy()
+    amps, mean_I = synth.poisson_amplitudes(I, np.random.default_rng(2))
+    rec = rc.Reconstruction(prob["positions"], amps, None, prob["probe"], (1,) + spec.obj_shape, mean_I,
+                            upsample_factor=spec.scan_step, device=dev)
+    hist = rec.reconstruct(8, 32, learn_probe=False, seed=0)
+    assert hist[-1] < 0.5 * hist[0]
+    est = rc.projected_phase(rec.obj_constrained.detach()).cpu().numpy()
+    crop = ((40, 88), (40, 88))
+    assert rc.phase_nmse(est, prob["true_phase"].sum(0), crop) < 0.5
