@@ -43,6 +43,10 @@ EXPORTED_SYMBOLS = (
     "qem_abi_version",
     "qem_profile_begin",
     "qem_profile_end",
+    "qem_pattern_features",
+    "qem_conv3x3_forward",
+    "qem_conv3x3_wgrad_workspace_bytes",
+    "qem_conv3x3_wgrad",
 )
 
 
@@ -85,6 +89,15 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.qem_profile_end.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int32),
                                     ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int32)]
     lib.qem_profile_end.restype = ctypes.c_int
+    i32 = ctypes.c_int32
+    lib.qem_pattern_features.argtypes = [i32, i32, i32, _vp, _vp, _vp, _vp]
+    lib.qem_pattern_features.restype = ctypes.c_int
+    lib.qem_conv3x3_forward.argtypes = [i32] * 5 + [_vp, _vp, _vp, i32, i32, _vp, _vp]
+    lib.qem_conv3x3_forward.restype = ctypes.c_int
+    lib.qem_conv3x3_wgrad_workspace_bytes.argtypes = [i32] * 5 + [ctypes.POINTER(ctypes.c_size_t)]
+    lib.qem_conv3x3_wgrad_workspace_bytes.restype = ctypes.c_int
+    lib.qem_conv3x3_wgrad.argtypes = [i32] * 5 + [_vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]
+    lib.qem_conv3x3_wgrad.restype = ctypes.c_int
     lib.qem_abi_version.argtypes = []
     lib.qem_abi_version.restype = ctypes.c_int32
 

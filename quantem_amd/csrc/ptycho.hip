@@ -21,6 +21,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include "common.h"
 #include "fft2.h"
 #include "qem_ptycho.h"
 
@@ -575,16 +576,30 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_fft2(const cf* in, cf* out, int 
 // C ABI
 // =============================================================================================
 namespace {
-
 thread_local char g_err[512] = "";
+}  // namespace
 
-int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
-int fail(int code, const char* fmt, ...) {
+int qem::set_error(int code, const char* fmt, ...) {
   va_list ap;
   va_start(ap, fmt);
   vsnprintf(g_err, sizeof(g_err), fmt, ap);
   va_end(ap);
   return code;
+}
+
+void qem::clear_error() { g_err[0] = 0; }
+
+int qem::check_last_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return qem::set_error(QEM_ERR_LAUNCH, "%s launch failed: %s", what, hipGetErrorString(e));
+  return QEM_OK;
+}
+
+namespace {
+
+template <typename... A>
+int fail(int code, const char* fmt, A... args) {
+  return qem::set_error(code, fmt, args...);
 }
 
 inline size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -666,11 +681,7 @@ int check_ptr(const void* ptr, const char* name, bool complex_) {
   return QEM_OK;
 }
 
-int check_launch(const char* what) {
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return fail(QEM_ERR_LAUNCH, "%s launch failed: %s", what, hipGetErrorString(e));
-  return QEM_OK;
-}
+int check_launch(const char* what) { return qem::check_last_launch(what); }
 
 qem::Params make_params(const qem_ptycho_desc& d, const Layout& L, void* ws) {
   qem::Params p;
@@ -751,7 +762,7 @@ extern "C" {
 int32_t qem_abi_version(void) { return 100; }
 
 int qem_profile_begin(int32_t max_launches) {
-  g_err[0] = 0;
+  qem::clear_error();
   if (g_prof.on) return fail(QEM_ERR_ARG, "profiling already active");
   if (max_launches < 1 || max_launches > 100000) return fail(QEM_ERR_ARG, "max_launches out of range");
   g_prof.cap = max_launches;
@@ -767,7 +778,7 @@ int qem_profile_begin(int32_t max_launches) {
 }
 
 int qem_profile_end(float* fwd_ms, int32_t* n_fwd, float* adj_ms, int32_t* n_adj) {
-  g_err[0] = 0;
+  qem::clear_error();
   if (!g_prof.on) return fail(QEM_ERR_ARG, "profiling not active");
   float tf = 0.f, ta = 0.f;
   int rc = QEM_OK;
@@ -802,7 +813,7 @@ int qem_profile_end(float* fwd_ms, int32_t* n_fwd, float* adj_ms, int32_t* n_adj
 const char* qem_last_error(void) { return g_err; }
 
 int qem_ptycho_workspace_bytes(const qem_ptycho_desc* desc, size_t* bytes) {
-  g_err[0] = 0;
+  qem::clear_error();
   if (int rc = check_desc(desc)) return rc;
   if (bytes == nullptr) return fail(QEM_ERR_NULL, "bytes is NULL");
   *bytes = layout_of(*desc).total;
@@ -812,7 +823,7 @@ int qem_ptycho_workspace_bytes(const qem_ptycho_desc* desc, size_t* bytes) {
 int qem_ptycho_forward_loss(const qem_ptycho_desc* desc, const void* obj, const void* probe, const float* positions,
                             const void* propagators, const float* targets, const float* mask, const float* descan,
                             float* loss, float* intensities, void* workspace, size_t workspace_bytes, void* stream) {
-  g_err[0] = 0;
+  qem::clear_error();
   if (int rc = check_desc(desc)) return rc;
   const qem_ptycho_desc& d = *desc;
   if (int rc = check_ptr(obj, "obj", true)) return rc;
@@ -854,7 +865,7 @@ int qem_ptycho_adjoint(const qem_ptycho_desc* desc, const void* obj, const void*
                        const void* propagators, const float* descan, const float* dloss, void* grad_obj,
                        void* grad_probe, float* grad_positions, float* grad_descan, void* workspace,
                        size_t workspace_bytes, void* stream) {
-  g_err[0] = 0;
+  qem::clear_error();
   if (int rc = check_desc(desc)) return rc;
   const qem_ptycho_desc& d = *desc;
   if (!(d.flags & QEM_FLAG_STORE_WAVES))
@@ -903,7 +914,7 @@ int qem_ptycho_adjoint(const qem_ptycho_desc* desc, const void* obj, const void*
 }
 
 int qem_fft2_c64(int32_t n, int32_t count, int32_t inverse, const void* in, void* out, void* stream) {
-  g_err[0] = 0;
+  qem::clear_error();
   if (n != 32 && n != 64 && n != 128) return fail(QEM_ERR_UNSUPPORTED, "fft size %d not supported", n);
   if (count < 0) return fail(QEM_ERR_ARG, "count must be >= 0");
   if (count == 0) return QEM_OK;

@@ -137,6 +137,7 @@ class Reconstruction:
         batcher = SimpleBatcher(self.num_gpts, batch_size, rng=seed)
         scale = self.upsample_factor ** 2 / 2
         t0 = time.perf_counter()
+        t_cb = 0.0  # time spent in callbacks, excluded from self.times
         for epoch in range(num_iters):
             total = torch.zeros((), dtype=torch.float32, device=self.device)
             for idx in batcher:
@@ -152,7 +153,9 @@ class Reconstruction:
                 total = total + loss.detach()
             mean_loss = float(total.item()) / len(batcher)  # one host sync per epoch
             self.history.append(mean_loss)
-            self.times.append(time.perf_counter() - t0)
+            self.times.append(time.perf_counter() - t0 - t_cb)
             if callback is not None:
+                tc = time.perf_counter()
                 callback(epoch, mean_loss, self)
+                t_cb += time.perf_counter() - tc
         return self.history

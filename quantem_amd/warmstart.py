@@ -1,0 +1,437 @@
+"""Learned object-phase warm start for multislice ptychography.
+
+Per-pattern features (centre of mass + 6 annular moments, the quantities of quantem's CoM fit,
+``dataset_models.py:1064-1074`` / ``origin_models.py:95-126``, and annular virtual detectors,
+``dataset4dstem.py:494-520``) are computed by a HIP kernel over the scan, giving an 8 x Sy x Sx image.
+A U-Net in the family of quantem's ``CNN2d`` (``core/ml/cnn.py:17-212``: 3x3 conv blocks with ReLU,
+``blocks.py:53-140``; 2x upsampling, ``blocks.py:143-210``) maps it to the projected phase over the
+scanned field of view.  Every 3x3 convolution (forward, input gradient, weight gradient) runs on the
+matrix cores through ``libqem_ptycho.so`` (bf16 operands, fp32 accumulation); pooling, upsampling,
+ReLU and concatenation are torch plumbing.  The network is trained only on pairs simulated by this
+package's multislice forward kernel, and the plain flat-object reconstruction remains the fallback
+and the judge of the warm start's error (:func:`evaluate`).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+import torch
+import torch.nn.functional as F
+
+from . import _native as nat
+from . import synth
+from .ptychography import forward_intensities
+from .reconstruct import Reconstruction, phase_nmse, projected_phase
+
+
+def _stream(dev) -> int:
+    return torch.cuda.current_stream(dev).cuda_stream
+
+
+# ---------------------------------------------------------------------------------------------
+# Features
+def pattern_features(patterns: torch.Tensor, edges: torch.Tensor, amplitudes: bool = True) -> torch.Tensor:
+    """(P, N, N) fftshifted patterns -> (P, 8): CoM (row, col) and 6 annular intensity fractions."""
+    if patterns.dtype != torch.float32 or patterns.dim() != 3:
+        raise TypeError("patterns must be float32 (P, N, N)")
+    if patterns.device.type != "cuda":
+        raise ValueError("patterns must be on a GPU")
+    pc = patterns.contiguous()
+    e = edges.to(device=pc.device, dtype=torch.float32).contiguous()
+    if e.numel() != 7:
+        raise ValueError("edges must hold 7 radii")
+    out = torch.empty((pc.shape[0], 8), dtype=torch.float32, device=pc.device)
+    rc = nat.load().qem_pattern_features(pc.shape[0], pc.shape[-1], int(amplitudes), pc.data_ptr(), e.data_ptr(),
+                                         out.data_ptr(), _stream(pc.device))
+    nat.check(rc, "qem_pattern_features")
+    return out
+
+
+def default_edges(bright_field_radius_px: float) -> torch.Tensor:
+    """Annuli around the bright-field disc: 0, .5, .8, 1, 1.2, 1.6, 2.5 bright-field radii."""
+    return torch.tensor([0.0, 0.5, 0.8, 1.0, 1.2, 1.6, 2.5]) * bright_field_radius_px
+
+
+# ---------------------------------------------------------------------------------------------
+# MFMA convolution
+def _to_nhwc_bf16(x: torch.Tensor) -> torch.Tensor:
+    return x.to(torch.bfloat16).contiguous(memory_format=torch.channels_last)
+
+
+def conv3x3_raw(x_bf16: torch.Tensor, w_khwc: torch.Tensor, bias: Optional[torch.Tensor], out_f32: bool) -> torch.Tensor:
+    """x: (N, C, H, W) bf16 channels_last; w: (Cout, 3, 3, Cin) bf16 contiguous -> (N, Cout, H, W)."""
+    nb, cin, h, w = x_bf16.shape
+    cout = w_khwc.shape[0]
+    y = torch.empty((nb, cout, h, w), dtype=torch.float32 if out_f32 else torch.bfloat16, device=x_bf16.device,
+                    memory_format=torch.channels_last)
+    b = None if bias is None else bias.to(torch.float32).contiguous()
+    rc = nat.load().qem_conv3x3_forward(nb, h, w, cin, cout, x_bf16.data_ptr(), w_khwc.data_ptr(),
+                                        None if b is None else b.data_ptr(), 0, int(out_f32), y.data_ptr(),
+                                        _stream(x_bf16.device))
+    nat.check(rc, "qem_conv3x3_forward")
+    return y
+
+
+def conv3x3_wgrad_raw(x_bf16: torch.Tensor, dy_bf16: torch.Tensor) -> torch.Tensor:
+    """dW (Cout, 3, 3, Cin) float32 for x (N, Cin, H, W), dy (N, Cout, H, W), both bf16 channels_last."""
+    nb, cin, h, w = x_bf16.shape
+    cout = dy_bf16.shape[1]
+    lib = nat.load()
+    need = ctypes.c_size_t(0)
+    nat.check(lib.qem_conv3x3_wgrad_workspace_bytes(nb, h, w, cin, cout, ctypes.byref(need)), "wgrad workspace")
+    ws = torch.empty(max(int(need.value), 4), dtype=torch.uint8, device=x_bf16.device)
+    dw = torch.empty((cout, 3, 3, cin), dtype=torch.float32, device=x_bf16.device)
+    rc = lib.qem_conv3x3_wgrad(nb, h, w, cin, cout, x_bf16.data_ptr(), dy_bf16.data_ptr(), dw.data_ptr(),
+                               ws.data_ptr(), ws.numel(), _stream(x_bf16.device))
+    nat.check(rc, "qem_conv3x3_wgrad")
+    return dw
+
+
+class Conv3x3(torch.autograd.Function):
+    """3x3 'same' convolution on MFMA: bf16 activations/weights, fp32 accumulation and gradients."""
+
+    @staticmethod
+    def forward(ctx, x, weight, bias, out_f32):
+        xb = _to_nhwc_bf16(x)
+        wb = weight.permute(0, 2, 3, 1).to(torch.bfloat16).contiguous()  # (Cout, 3, 3, Cin)
+        y = conv3x3_raw(xb, wb, bias, bool(out_f32))
+        ctx.save_for_backward(xb, weight)
+        ctx.need_dx = ctx.needs_input_grad[0]
+        ctx.x_dtype = x.dtype
+        return y
+
+    @staticmethod
+    def backward(ctx, dy):
+        xb, weight = ctx.saved_tensors
+        dyb = _to_nhwc_bf16(dy)
+        dx = None
+        if ctx.need_dx:
+            # dx = conv(dy, w') with w'[ci, ky, kx, co] = w[co, ci, 2-ky, 2-kx]
+            wt = weight.flip(2, 3).permute(1, 2, 3, 0).to(torch.bfloat16).contiguous()
+            dx = conv3x3_raw(dyb, wt, None, True)
+        dw = conv3x3_wgrad_raw(xb, dyb).permute(0, 3, 1, 2).contiguous()
+        db = dy.float().sum(dim=(0, 2, 3))
+        if dx is not None:
+            dx = dx.to(ctx.x_dtype)
+        return dx, dw, db, None
+
+
+class ConvBlock(torch.nn.Module):
+    def __init__(self, cin: int, cout: int, out_f32: bool = False, relu: bool = True):
+        super().__init__()
+        self.weight = torch.nn.Parameter(torch.empty(cout, cin, 3, 3))
+        self.bias = torch.nn.Parameter(torch.zeros(cout))
+        torch.nn.init.kaiming_normal_(self.weight, nonlinearity="relu")
+        self.out_f32 = out_f32
+        self.relu = relu
+
+    def forward(self, x):
+        y = Conv3x3.apply(x, self.weight, self.bias, self.out_f32)
+        return F.relu(y) if self.relu else y
+
+
+class WarmStartNet(torch.nn.Module):
+    """4-level U-Net (32-256 channels) on the scan grid + 2 upsampling stages to 4 px per scan step."""
+
+    def __init__(self, in_ch: int = 8, base: int = 32):
+        super().__init__()
+        c = [base, base * 2, base * 4, base * 8]
+        self.enc = torch.nn.ModuleList()
+        prev = in_ch
+        for ch in c:
+            self.enc.append(torch.nn.Sequential(ConvBlock(prev, ch), ConvBlock(ch, ch)))
+            prev = ch
+        self.dec = torch.nn.ModuleList()
+        for lvl in (2, 1, 0):
+            self.dec.append(torch.nn.Sequential(ConvBlock(prev + c[lvl], c[lvl]), ConvBlock(c[lvl], c[lvl])))
+            prev = c[lvl]
+        self.head = torch.nn.Sequential(ConvBlock(prev, 16), ConvBlock(16, 16))
+        self.out = ConvBlock(16, 16, out_f32=True, relu=False)  # channel 0 is the phase
+
+    def forward(self, feats: torch.Tensor) -> torch.Tensor:
+        x = feats
+        skips = []
+        for i, blk in enumerate(self.enc):
+            x = blk(x)
+            if i < 3:
+                skips.append(x)
+                x = F.avg_pool2d(x, 2)
+        for blk in self.dec:
+            x = F.interpolate(x, scale_factor=2, mode="nearest")
+            x = torch.cat([x, skips.pop().to(x.dtype)], dim=1)
+            x = blk(x)
+        x = F.interpolate(x, scale_factor=2, mode="nearest")
+        x = self.head[0](x)
+        x = F.interpolate(x, scale_factor=2, mode="nearest")
+        x = self.head[1](x)
+        return self.out(x)[:, 0]
+
+
+# ---------------------------------------------------------------------------------------------
+# Training pairs
+@dataclass
+class WarmStartSpec:
+    """The learned-warm-start configuration: 4 slices of 256^2, 128^2 patterns, 32x32 scan, 1e4 e-."""
+
+    num_slices: int = 4
+    obj_size: int = 256
+    roi: int = 128
+    scan: int = 32
+    step: float = 4.0
+    jitter: float = 0.5
+    density: tuple = (0.005, 0.02)
+    sigma: float = 1.5
+    amp: tuple = (0.1, 0.5)
+    dose: float = 1e4
+    slice_thickness: float = 2.0
+    energy: float = 300e3
+    sampling: float = 0.1
+    semiangle_mrad: float = 20.0
+    defocus: float = 100.0
+
+    @property
+    def fov(self) -> int:
+        """Side of the phase window the network predicts: 4 px per scan step."""
+        return int(round(self.scan * self.step))
+
+    @property
+    def origin(self) -> float:
+        ext = (self.scan - 1) * self.step
+        return (self.obj_size - ext) / 2.0
+
+    @property
+    def window(self) -> tuple:
+        r0 = int(round(self.origin - self.step / 2))
+        return (r0, r0 + self.fov)
+
+
+def atom_phase_fft(size: int, n_atoms: int, sigma: float, amp_range, rng: np.random.Generator) -> np.ndarray:
+    """Periodic Gaussian-atom phase by splatting weighted deltas and convolving in Fourier space."""
+    pos = rng.uniform(0, size, (n_atoms, 2))
+    amp = rng.uniform(amp_range[0], amp_range[1], n_atoms)
+    grid = np.zeros((size, size))
+    i0 = np.floor(pos).astype(int)
+    f = pos - i0
+    for dy in (0, 1):
+        wy = f[:, 0] if dy else 1 - f[:, 0]
+        for dx in (0, 1):
+            wx = f[:, 1] if dx else 1 - f[:, 1]
+            np.add.at(grid, ((i0[:, 0] + dy) % size, (i0[:, 1] + dx) % size), amp * wy * wx)
+    k = np.fft.fftfreq(size)
+    # transfer function of a Gaussian with unit peak: 2 pi sigma^2 exp(-2 pi^2 sigma^2 k^2)
+    g = 2 * np.pi * sigma**2 * np.exp(-2 * (np.pi * sigma) ** 2 * (k[:, None] ** 2 + k[None, :] ** 2))
+    return np.real(np.fft.ifft2(np.fft.fft2(grid) * g))
+
+
+class PairSimulator:
+    """Simulates (features, projected phase window) pairs with the multislice forward kernel."""
+
+    def __init__(self, spec: WarmStartSpec, device="cuda"):
+        self.spec = spec
+        self.dev = torch.device(device)
+        base = synth.real_space_probe(spec.roi, spec.sampling, spec.energy, spec.semiangle_mrad, spec.defocus)
+        self.probe_np = synth.mixed_probe(base, [1.0], spec.dose).astype(np.complex64)
+        self.probe = torch.from_numpy(self.probe_np).to(self.dev)
+        self.props_np = synth.fresnel_propagators(spec.roi, spec.sampling, spec.energy,
+                                                  [spec.slice_thickness] * (spec.num_slices - 1))
+        self.props = torch.from_numpy(self.props_np).to(self.dev) if spec.num_slices > 1 else None
+        lam = synth.electron_wavelength_angstrom(spec.energy)
+        dk = 1.0 / (spec.roi * spec.sampling)
+        self.edges = default_edges(spec.semiangle_mrad * 1e-3 / lam / dk).to(self.dev)
+
+    def object(self, seed: int) -> tuple:
+        s = self.spec
+        rng = np.random.default_rng(seed)
+        dens = rng.uniform(*s.density)
+        n_atoms = int(round(dens * s.obj_size * s.obj_size))
+        phases = np.stack([atom_phase_fft(s.obj_size, n_atoms, s.sigma, s.amp, rng) for _ in range(s.num_slices)])
+        pos = synth.raster_positions(s.scan, s.scan, s.step, s.jitter, rng, (s.origin, s.origin))
+        return phases, pos.astype(np.float32), rng
+
+    def simulate(self, seed: int) -> dict:
+        phases, pos, rng = self.object(seed)
+        obj = torch.from_numpy(np.exp(1j * phases).astype(np.complex64)).to(self.dev)
+        post = torch.from_numpy(pos).to(self.dev)
+        I = forward_intensities(obj, self.probe, post, self.props)
+        gen = torch.Generator(device=self.dev)
+        gen.manual_seed(int(seed) + 7)
+        counts = torch.poisson(I.clamp_min(0), generator=gen)
+        feats = pattern_features(counts, self.edges, amplitudes=False)
+        s = self.spec
+        f_img = feats.view(s.scan, s.scan, 8).permute(2, 0, 1).contiguous()
+        w0, w1 = s.window
+        target = phases.sum(0)[w0:w1, w0:w1]
+        return {"features": f_img, "target": torch.from_numpy(target.astype(np.float32)).to(self.dev),
+                "amplitudes": torch.sqrt(counts), "positions": pos, "phases": phases,
+                "mean_intensity": float(counts.sum(dim=(-2, -1)).mean())}
+
+
+def make_dataset(sim: PairSimulator, seeds) -> tuple:
+    feats, targets = [], []
+    for sd in seeds:
+        d = sim.simulate(int(sd))
+        feats.append(d["features"])
+        targets.append(d["target"])
+    return torch.stack(feats), torch.stack(targets)
+
+
+class FeatureNorm:
+    """Per-channel standardisation fitted on the training features."""
+
+    def __init__(self, feats: torch.Tensor):
+        self.mean = feats.mean(dim=(0, 2, 3), keepdim=True)
+        self.std = feats.std(dim=(0, 2, 3), keepdim=True).clamp_min(1e-6)
+
+    def __call__(self, f: torch.Tensor) -> torch.Tensor:
+        return (f - self.mean) / self.std
+
+
+def _centred(x: torch.Tensor) -> torch.Tensor:
+    return x - x.mean(dim=(-2, -1), keepdim=True)
+
+
+def train(net: WarmStartNet, feats: torch.Tensor, targets: torch.Tensor, norm: FeatureNorm, epochs: int = 20,
+          batch: int = 32, lr: float = 1e-3, seed: int = 0, log=None) -> list:
+    """Adam on the mean-removed phase MSE; returns per-epoch losses."""
+    opt = torch.optim.Adam(net.parameters(), lr=lr)
+    rng = np.random.default_rng(seed)
+    hist = []
+    n = feats.shape[0]
+    for ep in range(epochs):
+        perm = rng.permutation(n)
+        tot = torch.zeros((), device=feats.device)
+        for i in range(0, n, batch):
+            idx = torch.from_numpy(perm[i:i + batch]).to(feats.device)
+            pred = net(norm(feats[idx]))
+            loss = torch.mean((_centred(pred) - _centred(targets[idx])) ** 2)
+            opt.zero_grad(set_to_none=True)
+            loss.backward()
+            opt.step()
+            tot = tot + loss.detach() * len(idx)
+        hist.append(float(tot.item()) / n)
+        if log:
+            log(f"epoch {ep}: loss {hist[-1]:.5f}")
+    return hist
+
+
+def predict_window(net: WarmStartNet, norm: FeatureNorm, feats: torch.Tensor) -> torch.Tensor:
+    with torch.no_grad():
+        return net(norm(feats[None]))[0].float()
+
+
+# ---------------------------------------------------------------------------------------------
+# Evaluation: warm start vs flat start, NMSE against wall-clock
+def evaluate(net: WarmStartNet, norm: FeatureNorm, sim: PairSimulator, seed: int, epochs: int = 20,
+             batch: int = 64) -> dict:
+    """Runs the flat-init and the warm-started reconstruction of one held-out object.
+
+    Returns the NMSE-vs-time curves (time includes feature extraction and inference for the warm
+    start), the flat method's final NMSE and the warm start's time to reach it.
+    """
+    s = sim.spec
+    d = sim.simulate(seed)
+    true = d["phases"].sum(0)
+    w0, w1 = s.window
+    crop = ((w0, w1), (w0, w1))
+    shape = (s.num_slices, s.obj_size, s.obj_size)
+
+    def run(obj_init, t_offset):
+        rec = Reconstruction(d["positions"], d["amplitudes"], sim.props_np, sim.probe_np, shape, d["mean_intensity"],
+                             upsample_factor=s.step, obj_init=obj_init, device=sim.dev)
+        curve = []
+
+        def cb(ep, loss, r):
+            est = projected_phase(r.obj_constrained.detach()).cpu().numpy()
+            curve.append((t_offset + r.times[-1], phase_nmse(est, true, crop)))
+
+        est0 = projected_phase(rec.obj_constrained.detach()).cpu().numpy()
+        curve.append((t_offset, phase_nmse(est0, true, crop)))
+        rec.reconstruct(epochs, batch, learn_probe=False, seed=seed, callback=cb)
+        return curve
+
+    torch.cuda.synchronize()
+    flat = run(None, 0.0)
+    # warm start: features + inference + phase split over the slices
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    counts = d["amplitudes"] ** 2
+    f = pattern_features(counts, sim.edges, amplitudes=False).view(s.scan, s.scan, 8).permute(2, 0, 1)
+    phi = predict_window(net, norm, f)
+    init = torch.ones(shape, dtype=torch.complex64, device=sim.dev)
+    init[:, w0:w1, w0:w1] = torch.exp(1j * (phi / s.num_slices)).to(torch.complex64)
+    torch.cuda.synchronize()
+    t_init = time.perf_counter() - t0
+    warm = run(init, t_init)
+    target = flat[-1][1]
+    t_warm = next((t for t, e in warm if e <= target), math.inf)
+    return {"flat": flat, "warm": warm, "target_nmse": target, "t_flat": flat[-1][0], "t_warm_to_target": t_warm,
+            "ratio": t_warm / flat[-1][0] if flat[-1][0] > 0 else math.inf, "t_init": t_init}
+
+
+def main(argv=None) -> int:
+    """python -m quantem_amd.warmstart --train 400 --epochs 20 --eval 3"""
+    import argparse
+    import json
+    import sys
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--train", type=int, default=400, help="training objects (seeds 0..train-1)")
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--eval", type=int, default=3, help="held-out objects (seeds 100000+)")
+    ap.add_argument("--rec-epochs", type=int, default=20)
+    ap.add_argument("--out", type=str, default="")
+    args = ap.parse_args(argv)
+    dev = torch.device("cuda:0")
+
+    def log(m):
+        print(f"[warmstart] {m}", file=sys.stderr, flush=True)
+
+    spec = WarmStartSpec()
+    sim = PairSimulator(spec, dev)
+    t0 = time.perf_counter()
+    feats, targets = make_dataset(sim, range(args.train))
+    torch.cuda.synchronize()
+    t_sim = time.perf_counter() - t0
+    log(f"simulated {args.train} pairs in {t_sim:.1f} s")
+    norm = FeatureNorm(feats)
+    torch.manual_seed(0)
+    net = WarmStartNet().to(dev)
+    t0 = time.perf_counter()
+    hist = train(net, feats, targets, norm, epochs=args.epochs, batch=args.batch, lr=args.lr, log=log)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    log(f"trained {args.epochs} epochs in {t_train:.1f} s")
+    results = []
+    for k in range(args.eval):
+        r = evaluate(net, norm, sim, 100000 + k, epochs=args.rec_epochs)
+        log(f"seed {100000 + k}: flat final NMSE {r['target_nmse']:.4f} in {r['t_flat']:.2f} s; warm start "
+            f"initial NMSE {r['warm'][0][1]:.4f}, reaches it in {r['t_warm_to_target']:.2f} s (ratio {r['ratio']:.2f})")
+        results.append(r)
+    summary = {
+        "train_pairs": args.train, "train_epochs": args.epochs, "train_loss": hist, "t_simulate_s": t_sim,
+        "t_train_s": t_train,
+        "eval": [{"seed": 100000 + k, "flat_final_nmse": r["target_nmse"], "t_flat_s": r["t_flat"],
+                  "warm_initial_nmse": r["warm"][0][1], "warm_final_nmse": r["warm"][-1][1],
+                  "t_warm_to_target_s": r["t_warm_to_target"], "ratio": r["ratio"], "t_init_s": r["t_init"],
+                  "flat_curve": r["flat"], "warm_curve": r["warm"]} for k, r in enumerate(results)],
+        "median_ratio": float(np.median([r["ratio"] for r in results])) if results else None,
+    }
+    text = json.dumps(summary)
+    if args.out:
+        with open(args.out, "w") as f:
+            f.write(text)
+    print(json.dumps({k: v for k, v in summary.items() if k not in ("eval", "train_loss")} |
+                     {"ratios": [e["ratio"] for e in summary["eval"]]}), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
