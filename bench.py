@@ -124,7 +124,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--calibrate-copy", action="store_true",
-                    help="run one 1 GiB device-to-device copy first (reference point for HBM byte counters)")
+                    help="run one elementwise kernel over 1 GiB first (torch.neg: 1 GiB read, 1 GiB written), "
+                         "the reference point for the HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
@@ -140,9 +141,9 @@ def main() -> int:
         src = torch.ones(1 << 28, dtype=torch.float32, device=dev)  # 1 GiB
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
-        dst.copy_(src)
+        torch.neg(src, out=dst)
         torch.cuda.synchronize()
-        log("calibration copy: 1073741824 bytes read, 1073741824 bytes written")
+        log("calibration kernel torch.neg: 1073741824 bytes read, 1073741824 bytes written")
         del src, dst
     log(f"generating workload {spec.name}")
     data = make_inputs(spec, dev, args.seed)

@@ -32,7 +32,8 @@ LOSS_TYPES = {
     "poisson": 4,
 }
 
-SUPPORTED_ROI = (32, 64, 128)
+SUPPORTED_ROI = (32, 64, 128, 256)  # multislice pattern sizes
+FFT_ROI = (32, 64, 128)  # sizes of the standalone fft2 entry point
 
 EXPORTED_SYMBOLS = (
     "qem_ptycho_workspace_bytes",

@@ -23,93 +23,10 @@
 
 #include "common.h"
 #include "fft2.h"
+#include "ptycho_common.h"
 #include "qem_ptycho.h"
 
 namespace qem {
-
-struct Params {
-  int S, M, B, H, W, P, loss_type;
-  float mean_I;
-  unsigned flags;
-  const cf* obj;
-  const cf* probe;
-  const float* pos;
-  const cf* prop;
-  const float* targets;
-  const float* mask;
-  const float* descan;
-  float* loss_out;
-  float* intens_out;
-  const float* dloss;
-  cf* grad_obj;
-  cf* grad_probe;
-  float* grad_pos;
-  float* grad_descan;
-  // workspace
-  cf* psi;          // (B*M*S, N, N)  incident waves; reused by the adjoint for per-mode object-gradient patches
-  cf* X;            // (B*M, N, N)    far-field waves; reused by the adjoint for probe-gradient spectra
-  float* gI;        // (B, N, N)      dL/dI, unshifted
-  double* loss_part;  // per loss block
-  cf* pk;           // (M, N, N)      FFT2 of the probe modes
-  cf* gsum;         // (M, N, N)      sum over the batch of probe-gradient spectra
-  float* gpos_part;   // (B*M, 2)
-  float* gdesc_part;  // (B*M, 2)
-};
-
-__device__ __forceinline__ int pmod(int a, int n) {
-  int r = a % n;
-  return r < 0 ? r + n : r;
-}
-
-// numpy/torch fftfreq(N, d=1)[k] in float32 (exact for power-of-two N).
-template <int N>
-__device__ __forceinline__ float fftfreq1(int k) {
-  return (float)(k < N / 2 ? k : k - N) * (1.0f / (float)N);
-}
-
-// exp(-2 pi i x) for float32 x.
-__device__ __forceinline__ cf expm2pi(float x) {
-  float s, c;
-  sincospif(-2.0f * x, &s, &c);
-  return mk(c, s);
-}
-
-// Maps a linear block index to a work item so that consecutive items land on the same XCD
-// (blocks are dispatched round-robin over the 8 XCDs; each XCD has its own L2).
-__device__ __forceinline__ int xcd_item(int blk, int nblk) {
-  if ((nblk & 7) != 0) return blk;
-  return (blk & 7) * (nblk >> 3) + (blk >> 3);
-}
-
-template <int N>
-__device__ __forceinline__ void origin_and_shift(const float* pos, int b, int& r0, int& c0, float& dr, float& dc) {
-  const float pr = pos[2 * b], pc = pos[2 * b + 1];
-  const float rr = rintf(pr), rc = rintf(pc);  // round half to even, as torch.round
-  r0 = (int)rr;
-  c0 = (int)rc;
-  dr = pr - rr;
-  dc = pc - rc;
-}
-
-// Object element at a byte offset from the slice base.
-__device__ __forceinline__ cf obj_at(const cf* O, unsigned byte_off) {
-  return *(const cf*)((const char*)O + byte_off);
-}
-
-// Block-wide sum of one double per thread; result valid in thread 0.
-template <int T>
-__device__ __forceinline__ double block_sum(double x, double* red) {
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  if (l == 0) red[w] = x;
-  __syncthreads();
-  double t = 0.0;
-  if (threadIdx.x == 0) {
-    for (int i = 0; i < T / 64; ++i) t += red[i];
-  }
-  __syncthreads();
-  return t;
-}
 
 // ---------------------------------------------------------------------------------------------
 // Probe spectra: pk[m] = FFT2(probe[m]).
@@ -624,7 +541,7 @@ void prof_mark(bool adjoint_kernel, int edge, hipStream_t st) {
 }
 
 struct Layout {
-  size_t psi, X, gI, loss_part, pk, gsum, gpos, gdesc, total;
+  size_t psi, scratch, X, gI, loss_part, pk, gsum, gpos, gdesc, total;
   int nloss;
 };
 
@@ -634,7 +551,12 @@ Layout layout_of(const qem_ptycho_desc& d) {
   const size_t BM = (size_t)d.batch * d.num_modes;
   size_t o = 0;
   L.psi = o;
-  if (d.flags & QEM_FLAG_STORE_WAVES) o += align_up(BM * d.num_slices * N2 * 8);
+  if (d.flags & QEM_FLAG_STORE_WAVES)
+    o += align_up(BM * d.num_slices * N2 * 8);
+  else if (d.roi == 256)
+    o += align_up(BM * N2 * 8);  // one rolling wave per item
+  L.scratch = o;
+  if (d.roi == 256) o += align_up(BM * 2 * N2 * 8);
   L.X = o;
   o += align_up(BM * N2 * 8);
   L.gI = o;
@@ -656,8 +578,8 @@ Layout layout_of(const qem_ptycho_desc& d) {
 
 int check_desc(const qem_ptycho_desc* d) {
   if (d == nullptr) return fail(QEM_ERR_NULL, "desc is NULL");
-  if (d->roi != 32 && d->roi != 64 && d->roi != 128)
-    return fail(QEM_ERR_UNSUPPORTED, "roi %d not supported (32, 64 or 128)", d->roi);
+  if (d->roi != 32 && d->roi != 64 && d->roi != 128 && d->roi != 256)
+    return fail(QEM_ERR_UNSUPPORTED, "roi %d not supported (32, 64, 128 or 256)", d->roi);
   if (d->num_slices < 1) return fail(QEM_ERR_ARG, "num_slices must be >= 1 (got %d)", d->num_slices);
   if (d->num_modes < 1) return fail(QEM_ERR_ARG, "num_modes must be >= 1 (got %d)", d->num_modes);
   if (d->batch < 1) return fail(QEM_ERR_ARG, "batch must be >= 1 (got %d)", d->batch);
@@ -696,7 +618,8 @@ qem::Params make_params(const qem_ptycho_desc& d, const Layout& L, void* ws) {
   p.mean_I = d.mean_intensity;
   p.flags = d.flags;
   char* w = (char*)ws;
-  p.psi = (d.flags & QEM_FLAG_STORE_WAVES) ? (qem::cf*)(w + L.psi) : nullptr;
+  p.psi = ((d.flags & QEM_FLAG_STORE_WAVES) || d.roi == 256) ? (qem::cf*)(w + L.psi) : nullptr;
+  p.scratch = (d.roi == 256) ? (qem::cf*)(w + L.scratch) : nullptr;
   p.X = (qem::cf*)(w + L.X);
   p.gI = (float*)(w + L.gI);
   p.loss_part = (double*)(w + L.loss_part);
@@ -723,6 +646,45 @@ int launch_forward(const qem::Params& p, const Layout& L, hipStream_t st) {
   if (int rc = check_launch("loss")) return rc;
   hipLaunchKernelGGL(qem::k_loss_finalize, dim3(1), dim3(256), 0, st, p, L.nloss);
   return check_launch("loss_finalize");
+}
+
+int launch_forward256(const qem::Params& p, const Layout& L, hipStream_t st) {
+  prof_mark(false, 0, st);
+  if (int rc = qem::launch_chain256_forward(p, st)) return rc;
+  prof_mark(false, 1, st);
+  hipLaunchKernelGGL(qem::k_loss<256>, dim3(L.nloss), dim3(256), 0, st, p);
+  if (int rc = check_launch("loss")) return rc;
+  hipLaunchKernelGGL(qem::k_loss_finalize, dim3(1), dim3(256), 0, st, p, L.nloss);
+  return check_launch("loss_finalize");
+}
+
+int launch_reductions(const qem::Params& p, hipStream_t st);
+
+int launch_adjoint256(const qem::Params& p, hipStream_t st) {
+  prof_mark(true, 0, st);
+  if (int rc = qem::launch_chain256_adjoint(p, st)) return rc;
+  prof_mark(true, 1, st);
+  dim3 g((p.W + 31) / 32, (p.H + 31) / 32, p.S);
+  hipLaunchKernelGGL(qem::k_obj_reduce<256>, g, dim3(256), 0, st, p);
+  if (int rc = check_launch("obj_reduce")) return rc;
+  hipLaunchKernelGGL(qem::k_probe_reduce<256>, dim3(p.M * 256 * 256 / 64), dim3(256), 0, st, p);
+  if (int rc = check_launch("probe_reduce")) return rc;
+  if (int rc = qem::launch_probe_grad256(p, st)) return rc;
+  return launch_reductions(p, st);
+}
+
+int launch_reductions(const qem::Params& p, hipStream_t st) {
+  if (p.grad_pos != nullptr) {
+    hipLaunchKernelGGL(qem::k_pair_reduce, dim3((2 * p.B + 255) / 256), dim3(256), 0, st, p.gpos_part, p.grad_pos,
+                       p.B, p.M);
+    if (int rc = check_launch("position_reduce")) return rc;
+  }
+  if (p.grad_descan != nullptr) {
+    hipLaunchKernelGGL(qem::k_pair_reduce, dim3((2 * p.B + 255) / 256), dim3(256), 0, st, p.gdesc_part,
+                       p.grad_descan, p.B, p.M);
+    if (int rc = check_launch("descan_reduce")) return rc;
+  }
+  return QEM_OK;
 }
 
 template <int N>
@@ -857,7 +819,8 @@ int qem_ptycho_forward_loss(const qem_ptycho_desc* desc, const void* obj, const 
   switch (d.roi) {
     case 32: return launch_forward<32>(p, L, st);
     case 64: return launch_forward<64>(p, L, st);
-    default: return launch_forward<128>(p, L, st);
+    case 128: return launch_forward<128>(p, L, st);
+    default: return launch_forward256(p, L, st);
   }
 }
 
@@ -909,7 +872,8 @@ int qem_ptycho_adjoint(const qem_ptycho_desc* desc, const void* obj, const void*
   switch (d.roi) {
     case 32: return launch_adjoint<32>(p, st);
     case 64: return launch_adjoint<64>(p, st);
-    default: return launch_adjoint<128>(p, st);
+    case 128: return launch_adjoint<128>(p, st);
+    default: return launch_adjoint256(p, st);
   }
 }
 

@@ -1,0 +1,101 @@
+// ptycho_common.h — parameters and device helpers shared by the N <= 128 (register/LDS) and the
+// N = 256 (two-pass) multislice kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "fft2.h"
+#include "qem_ptycho.h"
+
+namespace qem {
+
+struct Params {
+  int S, M, B, H, W, P, loss_type;
+  float mean_I;
+  unsigned flags;
+  const cf* obj;
+  const cf* probe;
+  const float* pos;
+  const cf* prop;
+  const float* targets;
+  const float* mask;
+  const float* descan;
+  float* loss_out;
+  float* intens_out;
+  const float* dloss;
+  cf* grad_obj;
+  cf* grad_probe;
+  float* grad_pos;
+  float* grad_descan;
+  // workspace
+  cf* psi;          // (B*M*S, N, N)  incident waves; reused by the adjoint for per-mode object-gradient patches
+  cf* scratch;      // (B*M*2, N, N)  ping-pong buffers of the two-pass (N = 256) transforms
+  cf* X;            // (B*M, N, N)    far-field waves; reused by the adjoint for probe-gradient spectra
+  float* gI;        // (B, N, N)      dL/dI, unshifted
+  double* loss_part;  // per loss block
+  cf* pk;           // (M, N, N)      FFT2 of the probe modes
+  cf* gsum;         // (M, N, N)      sum over the batch of probe-gradient spectra
+  float* gpos_part;   // (B*M, 2)
+  float* gdesc_part;  // (B*M, 2)
+};
+
+__device__ __forceinline__ int pmod(int a, int n) {
+  int r = a % n;
+  return r < 0 ? r + n : r;
+}
+
+// numpy/torch fftfreq(N, d=1)[k] in float32 (exact for power-of-two N).
+template <int N>
+__device__ __forceinline__ float fftfreq1(int k) {
+  return (float)(k < N / 2 ? k : k - N) * (1.0f / (float)N);
+}
+
+// exp(-2 pi i x) for float32 x.
+__device__ __forceinline__ cf expm2pi(float x) {
+  float s, c;
+  sincospif(-2.0f * x, &s, &c);
+  return mk(c, s);
+}
+
+// Maps a linear block index to a work item so that consecutive items land on the same XCD
+// (blocks are dispatched round-robin over the 8 XCDs; each XCD has its own L2).
+__device__ __forceinline__ int xcd_item(int blk, int nblk) {
+  if ((nblk & 7) != 0) return blk;
+  return (blk & 7) * (nblk >> 3) + (blk >> 3);
+}
+
+template <int N>
+__device__ __forceinline__ void origin_and_shift(const float* pos, int b, int& r0, int& c0, float& dr, float& dc) {
+  const float pr = pos[2 * b], pc = pos[2 * b + 1];
+  const float rr = rintf(pr), rc = rintf(pc);  // round half to even, as torch.round
+  r0 = (int)rr;
+  c0 = (int)rc;
+  dr = pr - rr;
+  dc = pc - rc;
+}
+
+// Object element at a byte offset from the slice base.
+__device__ __forceinline__ cf obj_at(const cf* O, unsigned byte_off) {
+  return *(const cf*)((const char*)O + byte_off);
+}
+
+// Block-wide sum of one double per thread; result valid in thread 0.
+template <int T>
+__device__ __forceinline__ double block_sum(double x, double* red) {
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) red[w] = x;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < T / 64; ++i) t += red[i];
+  }
+  __syncthreads();
+  return t;
+}
+
+// Two-pass (N = 256) chain launchers (ptycho256.hip).
+int launch_chain256_forward(const Params& p, hipStream_t st);
+int launch_chain256_adjoint(const Params& p, hipStream_t st);
+int launch_probe_grad256(const Params& p, hipStream_t st);
+
+}  // namespace qem

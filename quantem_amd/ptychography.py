@@ -232,8 +232,8 @@ def forward_intensities(
 def _fft2(x: torch.Tensor, inverse: bool) -> torch.Tensor:
     if x.dtype != torch.complex64:
         raise TypeError(f"fft2 needs complex64, got {x.dtype}")
-    if x.dim() < 2 or x.shape[-1] != x.shape[-2] or x.shape[-1] not in nat.SUPPORTED_ROI:
-        raise ValueError(f"fft2 needs (..., N, N) with N in {nat.SUPPORTED_ROI}, got {tuple(x.shape)}")
+    if x.dim() < 2 or x.shape[-1] != x.shape[-2] or x.shape[-1] not in nat.FFT_ROI:
+        raise ValueError(f"fft2 needs (..., N, N) with N in {nat.FFT_ROI}, got {tuple(x.shape)}")
     if x.device.type != "cuda":
         raise ValueError(f"fft2 input must be on a GPU device, got {x.device}")
     xc = x.contiguous()
