@@ -33,6 +33,11 @@ from quantem_amd import synth  # noqa: E402
 METRIC = "patterns/s fwd+adjoint, % HBM roofline; time-to-target-NMSE warm-start vs flat"
 FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the adjoint kernel at this workload, from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE runs of this script (profiles/multislice_8x384_128px_4modes_hbm_traffic.md): FETCH_SIZE
+# 694836 KiB x 2.0 + WRITE_SIZE 1179650 KiB x 1.0, the factors being what the same counters read for
+# torch.neg over 1 GiB (FETCH_SIZE 524311 KiB, WRITE_SIZE 1048580 KiB).
+ADJOINT_TRAFFIC_BYTES = (694836 * 2.0 + 1179650 * 1.0) * 1024
 
 
 def log(msg: str) -> None:
@@ -70,13 +75,14 @@ def batches(P: int, B: int, seed: int) -> list:
 
 
 def gpu_pass(state: dict, data: dict, idx_list: list, scale: float) -> None:
-    obj, probe, opt = state["obj"], state["probe"], state["opt"]
+    obj_r, probe_r, opt = state["obj"], state["probe"], state["opt"]
     for idx in idx_list:
         opt.zero_grad(set_to_none=False)
+        obj, probe = torch.view_as_complex(obj_r), torch.view_as_complex(probe_r)
         loss = qp.multislice_loss(obj, probe, data["positions"][idx], data["propagators"], data["targets"][idx],
                                   None, None, num_gpts=data["num_gpts"], mean_intensity=data["mean_intensity"])
         loss.backward()
-        obj.grad.mul_(scale)
+        obj_r.grad.mul_(scale)
         opt.step()
 
 
@@ -149,9 +155,10 @@ def main() -> int:
     data = make_inputs(spec, dev, args.seed)
     S, (H, W), N, M, B = spec.num_slices, spec.obj_shape, spec.roi, spec.num_modes, spec.batch
     P = data["num_gpts"]
-    obj = torch.nn.Parameter(torch.ones((S, H, W), dtype=torch.complex64, device=dev))
-    probe = torch.nn.Parameter(data["probe"].clone())
-    opt = torch.optim.Adam([{"params": [obj], "lr": 5e-3}, {"params": [probe], "lr": 1e-3}])
+    # real (..., 2) storage so the single-kernel fused Adam applies; the update is the complex Adam's
+    obj = torch.nn.Parameter(torch.view_as_real(torch.ones((S, H, W), dtype=torch.complex64, device=dev)).clone())
+    probe = torch.nn.Parameter(torch.view_as_real(data["probe"]).clone())
+    opt = torch.optim.Adam([{"params": [obj], "lr": 5e-3}, {"params": [probe], "lr": 1e-3}], fused=True)
     state = {"obj": obj, "probe": probe, "opt": opt}
     upsample = ((np.asarray(spec.scan) - 1) * spec.scan_step / (np.asarray(spec.scan) - 1)).mean()
     scale = float(upsample**2 / 2)
@@ -176,14 +183,16 @@ def main() -> int:
     log(f"{ms_per_step:.3f} ms/step, {value:.0f} patterns/s; fwd kernel {kt.fwd_ms / max(kt.n_fwd, 1):.3f} ms x "
         f"{kt.n_fwd}, adjoint kernel {kt.adj_ms / max(kt.n_adj, 1):.3f} ms x {kt.n_adj}")
 
-    # roofline of the dominant kernel (FFT flops; see DESIGN.md for the unit counts)
+    # roofline of the dominant kernel: FFT flops, 5 N^2 log2(N^2) per 2-D transform, 2S transforms per (pattern, mode)
     flops_per_launch = B * M * 2 * S * fft_flops(N)
     t_fwd = kt.fwd_ms / max(kt.n_fwd, 1)
     t_adj = kt.adj_ms / max(kt.n_adj, 1)
     dom, t_dom = ("multislice adjoint kernel", t_adj) if t_adj >= t_fwd else ("multislice forward kernel", t_fwd)
     achieved = flops_per_launch / (t_dom / 1e3) / 1e12
     roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+            "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+            "traffic": ADJOINT_TRAFFIC_BYTES if dom == "multislice adjoint kernel" else None,
+            "traffic_gbs": round(ADJOINT_TRAFFIC_BYTES / (t_dom / 1e3) / 1e9, 1) if dom == "multislice adjoint kernel" else None,
             "kernel": dom, "kernel_ms": round(t_dom, 4),
             "fwd_kernel_ms": round(t_fwd, 4), "adj_kernel_ms": round(t_adj, 4),
             "note": "FP32 vector (FFT-flop) bound, no MFMA used; 157.3 TFLOP/s is the MI355X FP32 peak; "

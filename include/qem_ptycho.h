@@ -57,7 +57,8 @@ enum {
 enum {
   QEM_FLAG_STORE_WAVES = 1u << 0, /* keep the per-slice incident waves for qem_ptycho_adjoint */
   QEM_FLAG_GRAD_POSITIONS = 1u << 1, /* adjoint also writes d loss / d positions */
-  QEM_FLAG_GRAD_DESCAN = 1u << 2     /* adjoint also writes d loss / d descan */
+  QEM_FLAG_GRAD_DESCAN = 1u << 2,    /* adjoint also writes d loss / d descan */
+  QEM_FLAG_ANALYTIC = 1u << 3        /* set by qem_ptycho_projection_update (internal) */
 };
 
 typedef struct qem_ptycho_desc {
@@ -103,6 +104,22 @@ int qem_ptycho_adjoint(const qem_ptycho_desc* desc, const void* obj, const void*
                        const float* positions, const void* propagators, const float* descan,
                        const float* dloss, void* grad_obj, void* grad_probe, float* grad_positions,
                        float* grad_descan, void* workspace, size_t workspace_bytes, void* stream);
+
+/* The reference's analytical update (autograd=False, ptychography.py:339-372) for one batch, after a
+ * qem_ptycho_forward_loss call with QEM_FLAG_STORE_WAVES on the same descriptor, inputs and workspace:
+ *   Fourier projection of every mode's exit wave onto the measured amplitudes `targets`
+ *   (fourier_projection, ptychography.py:356-372), residual g = projected - exit wave (gradient_step,
+ *   350-354), reverse slice replay with conj(O_s) and conj propagators (ObjectPixelated.backward,
+ *   object_models.py:686-725).  Outputs (unnormalised; the caller forms the reference's updates):
+ *     obj_update     (S, Ho, Wo) complex64  sum over modes and patterns of conj(psi_s) g_s scattered
+ *     probe_update   (M, N, N)   complex64  sum over patterns of g at the probe plane
+ *     probe_overlap  (S, M, Ho, Wo) float32  sum over patterns of |psi_s,m|^2 scattered (or NULL)
+ *     patch_energy   (S, B)      float32  sum over each object patch of |O|^2 (or NULL)
+ * The stored waves are consumed: run qem_ptycho_forward_loss again before another call. */
+int qem_ptycho_projection_update(const qem_ptycho_desc* desc, const void* obj, const void* probe,
+                                 const float* positions, const void* propagators, const float* targets,
+                                 void* obj_update, void* probe_update, float* probe_overlap, float* patch_energy,
+                                 void* workspace, size_t workspace_bytes, void* stream);
 
 /* Batched 2-D complex64 FFT of `count` N x N arrays with torch.fft.fft2 / ifft2 default
  * normalisation (forward unscaled, inverse 1/N^2).  Replaces af.fft2 / af.ifft2

@@ -204,3 +204,47 @@ def adjoint(obj, probe, positions, propagators, targets, mask, num_gpts, mean_in
     out["loss"] = dloss * 0 + batch_loss(fw["I"], targets, mask, num_gpts, mean_intensity, loss_type)
     out["I"] = fw["I"]
     return out
+
+
+def analytical_update(obj, probe, positions, propagators, targets, descan=None, obj_type="complex"):
+    """The reference's autograd=False gradients (ptychography.py:339-372, object_models.py:686-725,
+    probe_models.py:774-783), float64: returns (obj_grad, probe_grad) as assigned to .grad."""
+    obj = np.asarray(obj, dtype=np.complex128)
+    probe = np.asarray(probe, dtype=np.complex128)
+    fw = forward(obj, probe, positions, propagators, descan)
+    n = probe.shape[-1]
+    M = probe.shape[0]
+    S = obj.shape[0]
+    amps = np.fft.fftshift(np.asarray(targets, dtype=np.float64), axes=(-2, -1))  # corner centred
+    X = np.fft.fft2(fw["exit"], norm="ortho")
+    if M == 1:
+        Xp = amps[None] * np.exp(1j * np.angle(X))
+    else:
+        a = np.sqrt(np.sum(np.abs(X + 1e-9) ** 2, axis=0))
+        a[a == 0] = np.inf
+        Xp = (amps / a)[None] * X
+    g = np.fft.ifft2(Xp, norm="ortho") - fw["exit"]
+    rows, cols = fw["rows"], fw["cols"]
+    obj_grad = np.zeros_like(obj)
+    props = None if propagators is None else np.asarray(propagators, dtype=np.complex128)
+    for s in range(S - 1, -1, -1):
+        psi = fw["psis"][s]
+        norm = 0.0
+        upd = np.zeros(obj.shape[1:], dtype=np.complex128)
+        for m in range(M):
+            ov = np.zeros(obj.shape[1:])
+            for b in range(psi.shape[1]):
+                np.add.at(ov, (rows[b][:, None], cols[b][None, :]), np.abs(psi[m, b]) ** 2)
+                term = np.conj(psi[m, b]) * g[m, b]
+                if obj_type == "potential":
+                    term = np.real(-1j * np.conj(fw["patches"][s][b]) * term)
+                np.add.at(upd, (rows[b][:, None], cols[b][None, :]), term)
+            norm += ov.max()
+        obj_grad[s] = upd / norm
+        g = g * np.conj(fw["patches"][s])[None]
+        if s > 0:
+            g = np.fft.ifft2(np.fft.fft2(g) * np.conj(props[s - 1]))
+    energy = np.sum(np.abs(fw["patches"]) ** 2, axis=(-2, -1)).max()
+    ortho_norm = 2 * n if M == 1 else 1 / (2 * n)
+    probe_grad = np.sum(g, axis=1) / energy / ortho_norm
+    return -obj_grad, -probe_grad

@@ -23,6 +23,7 @@ QEM_ERR_UNSUPPORTED = 6
 QEM_FLAG_STORE_WAVES = 1 << 0
 QEM_FLAG_GRAD_POSITIONS = 1 << 1
 QEM_FLAG_GRAD_DESCAN = 1 << 2
+QEM_FLAG_ANALYTIC = 1 << 3
 
 LOSS_TYPES = {
     "l2_amplitude": 0,
@@ -39,6 +40,7 @@ EXPORTED_SYMBOLS = (
     "qem_ptycho_workspace_bytes",
     "qem_ptycho_forward_loss",
     "qem_ptycho_adjoint",
+    "qem_ptycho_projection_update",
     "qem_fft2_c64",
     "qem_last_error",
     "qem_abi_version",
@@ -81,6 +83,8 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.qem_ptycho_forward_loss.restype = ctypes.c_int
     lib.qem_ptycho_adjoint.argtypes = [dp] + [_vp] * 11 + [ctypes.c_size_t, _vp]
     lib.qem_ptycho_adjoint.restype = ctypes.c_int
+    lib.qem_ptycho_projection_update.argtypes = [dp] + [_vp] * 10 + [ctypes.c_size_t, _vp]
+    lib.qem_ptycho_projection_update.restype = ctypes.c_int
     lib.qem_fft2_c64.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp, _vp, _vp]
     lib.qem_fft2_c64.restype = ctypes.c_int
     lib.qem_last_error.argtypes = []

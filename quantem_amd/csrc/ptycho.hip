@@ -202,13 +202,17 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
   int r0, c0;
   float dr, dc;
   origin_and_shift<N>(p.pos, b, r0, c0, dr, dc);
-  const float dl = p.dloss[0];
+  const float dl = (p.flags & QEM_FLAG_ANALYTIC) ? 1.0f : p.dloss[0];
   const float inv_n = 1.0f / (float)N;
   const float inv_n2 = 1.0f / (float)(N * N);
 
   cf v[PTS];
   cf* Xs = p.X + (size_t)item * N * N;
-  {
+  if (p.flags & QEM_FLAG_ANALYTIC) {
+    // the projection kernel left the Fourier-space residual X' - X over X: g = ifft2_ortho(residual)
+#pragma unroll
+    for (int e = 0; e < PTS; ++e) v[e] = cscale(Xs[f.p3_off(e)], inv_n);
+  } else {
     const float* gI = p.gI + (size_t)b * N * N;
     const float sc = 2.0f * dl * inv_n;
 #pragma unroll
@@ -285,42 +289,155 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
       }
     }
   };
-  for (int s = p.S - 1; s >= 1; --s) {
+  for (int s = p.S - 1; s >= 0; --s) {
     slice_step(s);
-    // g_x <- ifft2(fft2(g_psi) conj(prop_{s-1})), the adjoint of the forward propagation
-    f.fwd(v, lds);
-    const cf* __restrict__ pr = p.prop + (size_t)(s - 1) * N * N;
+    if (s > 0) {
+      // g_x <- ifft2(fft2(g_psi) conj(prop_{s-1})), the adjoint of the forward propagation
+      f.fwd(v, lds);
+      const cf* __restrict__ pr = p.prop + (size_t)(s - 1) * N * N;
 #pragma unroll
-    for (int e = 0; e < PTS; ++e) v[e] = cscale(cmulc(v[e], pr[f.p3_off(e)]), inv_n2);
-    f.inv(v, lds);
+      for (int e = 0; e < PTS; ++e) v[e] = cscale(cmulc(v[e], pr[f.p3_off(e)]), inv_n2);
+      f.inv(v, lds);
+    }
   }
-  slice_step(0);
   f.fwd(v, lds);
-  // v = G = fft2(g_psi_0) in P3.
-  const cf rc = expm2pi(fftfreq1<N>(f.kc) * dc);
-  const cf* pk = p.pk + (size_t)m * N * N;
-  const bool want_pos = (p.flags & QEM_FLAG_GRAD_POSITIONS) && (p.gpos_part != nullptr);
-  double ar = 0.0, ac = 0.0;
-  const float fc = fftfreq1<N>(f.kc);
+  // G = fft2(g_psi_0), natural order, over X (the epilogue kernel turns it into the probe-gradient
+  // spectrum and the position-gradient partials)
 #pragma unroll
-  for (int e = 0; e < PTS; ++e) {
-    const int kr = f.p3_row(e);
-    const float fr = fftfreq1<N>(kr);
-    const cf ramp = cmul(expm2pi(fr * dr), rc);
-    const int k = f.p3_off(e);
-    Xs[k] = cmulc(v[e], ramp);
-    const float im = cconjmul(v[e], cmul(pk[k], ramp)).y;  // cheap; kept branch-free
+  for (int e = 0; e < PTS; ++e) Xs[f.p3_off(e)] = v[e];
+}
+
+// Adjoint epilogue, elementwise on G = fft2(g_psi_0) of one (pattern, mode) per block:
+//   spectrum conj(ramp_b) G (over G), position-gradient partials (2 pi / N^2) sum f Im(conj(G) P_m ramp_b).
+template <int N>
+__global__ __launch_bounds__(256) void k_adjoint_epilogue(Params p) {
+  __shared__ double red[4];
+  const int item = blockIdx.x;
+  const int b = item / p.M, m = item - b * p.M;
+  int r0, c0;
+  float dr, dc;
+  origin_and_shift<N>(p.pos, b, r0, c0, dr, dc);
+  cf* Xs = p.X + (size_t)item * N * N;
+  const cf* pk = p.pk + (size_t)m * N * N;
+  if (p.flags & QEM_FLAG_ANALYTIC) return;  // the reference sums g at the probe plane without the shift's adjoint
+  double ar = 0.0, ac = 0.0;
+  for (int idx = threadIdx.x; idx < N * N; idx += 256) {
+    const int kr = idx / N, kc = idx - (idx / N) * N;
+    const float fr = fftfreq1<N>(kr), fc = fftfreq1<N>(kc);
+    const cf ramp = cmul(expm2pi(fr * dr), expm2pi(fc * dc));
+    const cf G = Xs[idx];
+    Xs[idx] = cmulc(G, ramp);
+    const float im = cconjmul(G, cmul(pk[idx], ramp)).y;
     ar += (double)(fr * im);
     ac += (double)(fc * im);
   }
-  if (want_pos) {
-    const double tr = block_sum<F::T>(ar, red);
-    const double tc = block_sum<F::T>(ac, red);
+  if (p.flags & QEM_FLAG_GRAD_POSITIONS) {
+    const double tr = block_sum<256>(ar, red);
+    const double tc = block_sum<256>(ac, red);
     if (threadIdx.x == 0) {
-      p.gpos_part[2 * item] = (float)(2.0 * M_PI * tr * (double)inv_n2);
-      p.gpos_part[2 * item + 1] = (float)(2.0 * M_PI * tc * (double)inv_n2);
+      p.gpos_part[2 * item] = (float)(2.0 * M_PI * tr / (double)(N * N));
+      p.gpos_part[2 * item + 1] = (float)(2.0 * M_PI * tc / (double)(N * N));
     }
   }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Fourier projection residual of the analytical update (ptychography.py:356-372), over X:
+//   one mode:  X' = A exp(i angle(X))             -> X' - X
+//   M modes:   X'_m = (A / a) X_m, a = sqrt(sum_m |X_m + 1e-9|^2), A / a = 0 where a = 0
+// with A the measured amplitude at the fftshifted pixel.
+template <int N>
+__global__ __launch_bounds__(256) void k_projection(Params p) {
+  const int idx = blockIdx.x * 256 + threadIdx.x;  // over B*N*N
+  const int b = idx / (N * N);
+  const int k = idx - b * N * N;
+  const int kr = k / N, kc = k - kr * N;
+  const int ks = ((kr + N / 2) & (N - 1)) * N + ((kc + N / 2) & (N - 1));
+  const float A = p.targets[(size_t)b * N * N + ks];
+  cf* X = p.X + (size_t)b * p.M * N * N + k;
+  if (p.M == 1) {
+    const cf x = X[0];
+    const float mag = sqrtf(cabs2(x));
+    X[0] = mag > 0.0f ? mk(x.x * (A / mag) - x.x, x.y * (A / mag) - x.y) : mk(A - x.x, -x.y);
+  } else {
+    float s2 = 0.0f;
+    for (int m = 0; m < p.M; ++m) {
+      const cf x = X[(size_t)m * N * N];
+      s2 += (x.x + 1e-9f) * (x.x + 1e-9f) + x.y * x.y;
+    }
+    const float a = sqrtf(s2);
+    const float r = a > 0.0f ? A / a : 0.0f;
+    for (int m = 0; m < p.M; ++m) {
+      const cf x = X[(size_t)m * N * N];
+      X[(size_t)m * N * N] = mk(x.x * (r - 1.0f), x.y * (r - 1.0f));
+    }
+  }
+}
+
+// Per-mode probe overlap sum_b |psi_s,m,b|^2 scattered into the object frame (the normalisation of
+// ObjectPixelated.backward, object_models.py:704-706); same tiling as k_obj_reduce, one mode per
+// blockIdx.z % M.  Output (S, M, H, W) float32.
+template <int N>
+__global__ __launch_bounds__(256) void k_overlap(Params p, float* out) {
+  __shared__ int su0[256], sv0[256];
+  const int s = blockIdx.z / p.M, m = blockIdx.z - (blockIdx.z / p.M) * p.M;
+  const int X0 = blockIdx.x * 32, Y0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int x = X0 + tx;
+  float acc[4] = {0.f, 0.f, 0.f, 0.f};
+  for (int b0 = 0; b0 < p.B; b0 += 256) {
+    const int nb = min(256, p.B - b0);
+    if ((int)threadIdx.x < nb) {
+      const int bb = b0 + threadIdx.x;
+      const int r0 = (int)rintf(p.pos[2 * bb]), c0 = (int)rintf(p.pos[2 * bb + 1]);
+      const int u0 = pmod(Y0 - r0 + N / 2, p.H), v0 = pmod(X0 - c0 + N / 2, p.W);
+      const bool hit = (u0 < N || u0 + 31 >= p.H) && (v0 < N || v0 + 31 >= p.W);
+      su0[threadIdx.x] = hit ? u0 : -1;
+      sv0[threadIdx.x] = v0;
+    }
+    __syncthreads();
+    for (int i = 0; i < nb; ++i) {
+      const int u0 = su0[i];
+      if (u0 < 0) continue;
+      int ux = sv0[i] + tx;
+      if (ux >= p.W) ux -= p.W;
+      if (x >= p.W || ux >= N) continue;
+      const int col = (ux + N / 2) & (N - 1);
+      const cf* g = p.psi + (((size_t)(b0 + i) * p.M + m) * p.S + s) * N * N + col;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int dy = ty + 8 * j;
+        int uy = u0 + dy;
+        if (uy >= p.H) uy -= p.H;
+        if (Y0 + dy < p.H && uy < N) acc[j] += cabs2(g[((uy + N / 2) & (N - 1)) * N]);
+      }
+    }
+    __syncthreads();
+  }
+  if (x >= p.W) return;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int y = Y0 + ty + 8 * j;
+    if (y < p.H) out[(((size_t)s * p.M + m) * p.H + y) * p.W + x] = acc[j];
+  }
+}
+
+// Patch energy sum |O_s[patch b]|^2 (ProbePixelated.backward's normalisation, probe_models.py:775),
+// one block per (s, b); output (S, B) float32.
+template <int N>
+__global__ __launch_bounds__(256) void k_patch_energy(Params p, float* out) {
+  __shared__ double red[4];
+  const int s = blockIdx.x / p.B, b = blockIdx.x - (blockIdx.x / p.B) * p.B;
+  const int r0 = (int)rintf(p.pos[2 * b]), c0 = (int)rintf(p.pos[2 * b + 1]);
+  const cf* O = p.obj + (size_t)s * p.H * p.W;
+  double acc = 0.0;
+  for (int idx = threadIdx.x; idx < N * N; idx += 256) {
+    const int r = idx / N, c = idx - (idx / N) * N;
+    const int y = pmod(r0 + (r < N / 2 ? r : r - N), p.H), xx = pmod(c0 + (c < N / 2 ? c : c - N), p.W);
+    acc += (double)cabs2(O[(size_t)y * p.W + xx]);
+  }
+  const double t = block_sum<256>(acc, red);
+  if (threadIdx.x == 0) out[(size_t)s * p.B + b] = (float)t;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -514,10 +631,13 @@ int qem::check_last_launch(const char* what) {
 
 namespace {
 
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wformat-security"
 template <typename... A>
 int fail(int code, const char* fmt, A... args) {
   return qem::set_error(code, fmt, args...);
 }
+#pragma clang diagnostic pop
 
 inline size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
@@ -697,6 +817,8 @@ int launch_adjoint(const qem::Params& p, hipStream_t st) {
     hipLaunchKernelGGL((qem::k_adjoint<N, false>), dim3(p.B * p.M), dim3(F::T), 0, st, p);
   prof_mark(true, 1, st);
   if (int rc = check_launch("adjoint")) return rc;
+  hipLaunchKernelGGL(qem::k_adjoint_epilogue<N>, dim3(p.B * p.M), dim3(256), 0, st, p);
+  if (int rc = check_launch("adjoint_epilogue")) return rc;
   dim3 g((p.W + 31) / 32, (p.H + 31) / 32, p.S);
   hipLaunchKernelGGL(qem::k_obj_reduce<N>, g, dim3(256), 0, st, p);
   if (int rc = check_launch("obj_reduce")) return rc;
@@ -715,6 +837,31 @@ int launch_adjoint(const qem::Params& p, hipStream_t st) {
     if (int rc = check_launch("descan_reduce")) return rc;
   }
   return QEM_OK;
+}
+
+template <int N>
+int launch_projection_update(const qem::Params& p, const Layout& L, float* overlap, float* energy, hipStream_t st) {
+  using F = qem::Fft2<N>;
+  hipLaunchKernelGGL(qem::k_projection<N>, dim3(L.nloss), dim3(256), 0, st, p);
+  if (int rc = check_launch("projection")) return rc;
+  if (overlap != nullptr) {
+    dim3 g((p.W + 31) / 32, (p.H + 31) / 32, p.S * p.M);
+    hipLaunchKernelGGL(qem::k_overlap<N>, g, dim3(256), 0, st, p, overlap);
+    if (int rc = check_launch("overlap")) return rc;
+  }
+  if (energy != nullptr) {
+    hipLaunchKernelGGL(qem::k_patch_energy<N>, dim3(p.S * p.B), dim3(256), 0, st, p, energy);
+    if (int rc = check_launch("patch_energy")) return rc;
+  }
+  hipLaunchKernelGGL((qem::k_adjoint<N, false>), dim3(p.B * p.M), dim3(F::T), 0, st, p);
+  if (int rc = check_launch("adjoint")) return rc;
+  dim3 g((p.W + 31) / 32, (p.H + 31) / 32, p.S);
+  hipLaunchKernelGGL(qem::k_obj_reduce<N>, g, dim3(256), 0, st, p);
+  if (int rc = check_launch("obj_reduce")) return rc;
+  hipLaunchKernelGGL(qem::k_probe_reduce<N>, dim3(p.M * N * N / 64), dim3(256), 0, st, p);
+  if (int rc = check_launch("probe_reduce")) return rc;
+  hipLaunchKernelGGL(qem::k_probe_grad<N>, dim3(p.M), dim3(F::T), 0, st, p);
+  return check_launch("probe_grad");
 }
 
 }  // namespace
@@ -874,6 +1021,50 @@ int qem_ptycho_adjoint(const qem_ptycho_desc* desc, const void* obj, const void*
     case 64: return launch_adjoint<64>(p, st);
     case 128: return launch_adjoint<128>(p, st);
     default: return launch_adjoint256(p, st);
+  }
+}
+
+int qem_ptycho_projection_update(const qem_ptycho_desc* desc, const void* obj, const void* probe,
+                                 const float* positions, const void* propagators, const float* targets,
+                                 void* obj_update, void* probe_update, float* probe_overlap, float* patch_energy,
+                                 void* workspace, size_t workspace_bytes, void* stream) {
+  qem::clear_error();
+  if (int rc = check_desc(desc)) return rc;
+  qem_ptycho_desc d = *desc;
+  if (!(d.flags & QEM_FLAG_STORE_WAVES))
+    return fail(QEM_ERR_ARG, "projection update needs a forward pass run with QEM_FLAG_STORE_WAVES");
+  if (d.roi == 256) return fail(QEM_ERR_UNSUPPORTED, "projection update is implemented for roi <= 128");
+  if (int rc = check_ptr(obj, "obj", true)) return rc;
+  if (int rc = check_ptr(probe, "probe", true)) return rc;
+  if (int rc = check_ptr(positions, "positions", false)) return rc;
+  if (d.num_slices > 1)
+    if (int rc = check_ptr(propagators, "propagators", true)) return rc;
+  if (int rc = check_ptr(targets, "targets", false)) return rc;
+  if (int rc = check_ptr(obj_update, "obj_update", true)) return rc;
+  if (int rc = check_ptr(probe_update, "probe_update", true)) return rc;
+  if (probe_overlap != nullptr)
+    if (int rc = check_ptr(probe_overlap, "probe_overlap", false)) return rc;
+  if (patch_energy != nullptr)
+    if (int rc = check_ptr(patch_energy, "patch_energy", false)) return rc;
+  if (int rc = check_ptr(workspace, "workspace", true)) return rc;
+  const Layout L = layout_of(d);
+  if (workspace_bytes < L.total)
+    return fail(QEM_ERR_WORKSPACE, "workspace has %zu bytes, needs %zu", workspace_bytes, L.total);
+  d.flags |= QEM_FLAG_ANALYTIC;
+  d.flags &= ~(uint32_t)(QEM_FLAG_GRAD_POSITIONS | QEM_FLAG_GRAD_DESCAN);
+  qem::Params p = make_params(d, L, workspace);
+  p.obj = (const qem::cf*)obj;
+  p.probe = (const qem::cf*)probe;
+  p.pos = positions;
+  p.prop = (const qem::cf*)propagators;
+  p.targets = targets;
+  p.grad_obj = (qem::cf*)obj_update;
+  p.grad_probe = (qem::cf*)probe_update;
+  hipStream_t st = (hipStream_t)stream;
+  switch (d.roi) {
+    case 32: return launch_projection_update<32>(p, L, probe_overlap, patch_energy, st);
+    case 64: return launch_projection_update<64>(p, L, probe_overlap, patch_energy, st);
+    default: return launch_projection_update<128>(p, L, probe_overlap, patch_energy, st);
   }
 }
 

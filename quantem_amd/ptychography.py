@@ -253,3 +253,66 @@ def fft2(x: torch.Tensor) -> torch.Tensor:
 def ifft2(x: torch.Tensor) -> torch.Tensor:
     """torch.fft.ifft2 (default norm, 1/N^2) over the last two dims, N in {32, 64, 128}."""
     return _fft2(x, True)
+
+
+def analytical_gradients(
+    obj: torch.Tensor,
+    probe: torch.Tensor,
+    positions: torch.Tensor,
+    propagators: Optional[torch.Tensor],
+    targets: torch.Tensor,
+    mask: Optional[torch.Tensor] = None,
+    descan: Optional[torch.Tensor] = None,
+    *,
+    num_gpts: int,
+    mean_intensity: float,
+    loss_type: str = "l2_amplitude",
+    obj_type: str = "complex",
+):
+    """The reference's autograd=False update for one batch (ptychography.py:339-372).
+
+    Returns ``(loss, obj_grad, probe_grad)`` with the values the reference assigns to
+    ``_obj.grad`` (``object_models.py:686-725``: minus the slice-replayed update divided by the summed
+    per-mode maximum probe overlap) and ``_probe.grad`` (``probe_models.py:774-783``: minus the batch sum
+    of the probe-plane gradient over the maximum patch energy and its ``ortho_norm``).  ``obj`` is the
+    transmission; for ``obj_type="potential"`` the update is ``real(-i conj(O) ...)`` as in the reference.
+    """
+    device = obj.device
+    S, H, W = obj.shape
+    M, N, _ = probe.shape
+    if N == 256:
+        raise ValueError("the analytical update is implemented for roi <= 128")
+    B = positions.shape[0]
+    obj_c = _need(obj.detach(), "obj", torch.complex64, (S, H, W), device)
+    probe_c = _need(probe.detach(), "probe", torch.complex64, (M, N, N), device)
+    pos_c = _need(positions.detach(), "positions", torch.float32, (B, 2), device)
+    prop_c = _need(propagators, "propagators", torch.complex64, (S - 1, N, N), device) if S > 1 else None
+    tgt_c = _need(targets, "targets", torch.float32, (B, N, N), device)
+    mask_c = torch.ones((N, N), dtype=torch.float32, device=device) if mask is None else \
+        _need(mask, "mask", torch.float32, (N, N), device)
+    desc_c = None if descan is None else _need(descan.detach(), "descan", torch.float32, (B, 2), device)
+    d = _desc(S, M, B, N, H, W, num_gpts, loss_type, mean_intensity, nat.QEM_FLAG_STORE_WAVES)
+    ws = torch.empty(workspace_bytes(d), dtype=torch.uint8, device=device)
+    loss = torch.empty((), dtype=torch.float32, device=device)
+    lib = nat.load()
+    st = _stream(device)
+    nat.check(lib.qem_ptycho_forward_loss(ctypes.byref(d), _ptr(obj_c), _ptr(probe_c), _ptr(pos_c), _ptr(prop_c),
+                                          _ptr(tgt_c), _ptr(mask_c), _ptr(desc_c), _ptr(loss), None, _ptr(ws),
+                                          ws.numel(), st), "qem_ptycho_forward_loss")
+    obj_upd = torch.empty_like(obj_c)
+    probe_upd = torch.empty_like(probe_c)
+    overlap = torch.empty((S, M, H, W), dtype=torch.float32, device=device)
+    energy = torch.empty((S, B), dtype=torch.float32, device=device)
+    nat.check(lib.qem_ptycho_projection_update(ctypes.byref(d), _ptr(obj_c), _ptr(probe_c), _ptr(pos_c),
+                                               _ptr(prop_c), _ptr(tgt_c), _ptr(obj_upd), _ptr(probe_upd),
+                                               _ptr(overlap), _ptr(energy), _ptr(ws), ws.numel(), st),
+              "qem_ptycho_projection_update")
+    probe_norm = overlap.amax(dim=(-2, -1)).sum(dim=1)  # (S,) sum over modes of the max overlap
+    if obj_type == "potential":
+        upd = torch.real(-1j * torch.conj(obj_c) * obj_upd)
+    else:
+        upd = obj_upd
+    obj_grad = -upd / probe_norm.view(S, 1, 1)
+    ortho_norm = 2.0 * N if M == 1 else 1.0 / (2.0 * N)
+    probe_grad = -probe_upd / energy.max() / ortho_norm
+    return loss, obj_grad, probe_grad

@@ -19,7 +19,11 @@ from typing import Callable, Optional
 import numpy as np
 import torch
 
-from .ptychography import multislice_loss
+from .ptychography import analytical_gradients, multislice_loss
+
+
+def _real_view(x: torch.Tensor) -> torch.Tensor:
+    return torch.view_as_real(x) if x.is_complex() else x
 
 
 class SimpleBatcher:
@@ -107,14 +111,28 @@ class Reconstruction:
         self.obj_type = obj_type
         self.orthogonalize = orthogonalize
         self.upsample_factor = float(upsample_factor)
+        # a potential object is real and starts at zero; the others are complex and start flat at 1
+        odt = torch.float32 if obj_type == "potential" else torch.complex64
         if obj_init is None:
-            obj0 = torch.ones(tuple(int(x) for x in obj_shape), dtype=torch.complex64, device=dev)
+            obj0 = (torch.zeros if obj_type == "potential" else torch.ones)(
+                tuple(int(x) for x in obj_shape), dtype=odt, device=dev)
         else:
-            obj0 = torch.as_tensor(obj_init, dtype=torch.complex64, device=dev).clone()
-        self.obj = torch.nn.Parameter(obj0)
-        self.probe = torch.nn.Parameter(torch.as_tensor(probe, dtype=torch.complex64, device=dev).clone())
+            obj0 = torch.as_tensor(obj_init, dtype=odt, device=dev).clone()
+        # complex parameters live in real (..., 2) storage so that the fused Adam kernel applies
+        # (it is the complex Adam's update, which works on the same real view)
+        self._obj = torch.nn.Parameter(_real_view(obj0).clone())
+        self._probe = torch.nn.Parameter(
+            _real_view(torch.as_tensor(probe, dtype=torch.complex64, device=dev)).clone())
         self.history: list = []
         self.times: list = []
+
+    @property
+    def obj(self) -> torch.Tensor:
+        return self._obj if self.obj_type == "potential" else torch.view_as_complex(self._obj)
+
+    @property
+    def probe(self) -> torch.Tensor:
+        return torch.view_as_complex(self._probe)
 
     @property
     def obj_constrained(self) -> torch.Tensor:
@@ -126,14 +144,19 @@ class Reconstruction:
 
     def reconstruct(self, num_iters: int, batch_size: int, obj_lr: float = 5e-3, probe_lr: float = 1e-3,
                     learn_probe: bool = True, loss_type: str = "l2_amplitude", seed: int = 0,
+                    autograd: bool = True,
                     callback: Optional[Callable[[int, float, "Reconstruction"], None]] = None) -> list:
-        """Runs `num_iters` epochs; returns the per-epoch mean batch loss."""
-        groups = [{"params": [self.obj], "lr": obj_lr}]
+        """Runs `num_iters` epochs; returns the per-epoch mean batch loss.
+
+        ``autograd=False`` selects the reference's analytical update (Fourier projection and slice
+        replay, ``ptychography.py:339-372``) instead of the loss gradient; as in the reference, its
+        gradients are assigned to the raw parameters without the upsample rescale.
+        """
+        groups = [{"params": [self._obj], "lr": obj_lr}]
         if learn_probe:
-            groups.append({"params": [self.probe], "lr": probe_lr})
-        else:
-            self.probe.requires_grad_(False)
-        opt = torch.optim.Adam(groups)
+            groups.append({"params": [self._probe], "lr": probe_lr})
+        self._probe.requires_grad_(learn_probe)
+        opt = torch.optim.Adam(groups, fused=self.device.type == "cuda")
         batcher = SimpleBatcher(self.num_gpts, batch_size, rng=seed)
         scale = self.upsample_factor ** 2 / 2
         t0 = time.perf_counter()
@@ -143,12 +166,22 @@ class Reconstruction:
             for idx in batcher:
                 it = torch.from_numpy(idx).to(self.device)
                 opt.zero_grad(set_to_none=False)
-                loss = multislice_loss(self.obj_constrained, self.probe_constrained, self.positions[it],
-                                       self.propagators, self.targets[it], self.mask, None,
-                                       num_gpts=self.num_gpts, mean_intensity=self.mean_intensity,
-                                       loss_type=loss_type)
-                loss.backward()
-                self.obj.grad.mul_(scale)
+                if autograd:
+                    loss = multislice_loss(self.obj_constrained, self.probe_constrained, self.positions[it],
+                                           self.propagators, self.targets[it], self.mask, None,
+                                           num_gpts=self.num_gpts, mean_intensity=self.mean_intensity,
+                                           loss_type=loss_type)
+                    loss.backward()
+                    self._obj.grad.mul_(scale)
+                else:
+                    with torch.no_grad():
+                        loss, og, pg = analytical_gradients(
+                            self.obj_constrained, self.probe_constrained, self.positions[it], self.propagators,
+                            self.targets[it], self.mask, None, num_gpts=self.num_gpts,
+                            mean_intensity=self.mean_intensity, loss_type=loss_type, obj_type=self.obj_type)
+                    self._obj.grad = _real_view(og).contiguous()
+                    if learn_probe:
+                        self._probe.grad = _real_view(pg).contiguous()
                 opt.step()
                 total = total + loss.detach()
             mean_loss = float(total.item()) / len(batcher)  # one host sync per epoch

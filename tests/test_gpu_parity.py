@@ -195,3 +195,19 @@ def test_argument_errors(cuda):
         qp.multislice_loss(obj, probe, pos, props, tg, loss_type="l3", **args)
     with pytest.raises(ValueError):
         qp.multislice_loss(obj, probe, pos, props, tg, num_gpts=8, mean_intensity=0.0)
+
+
+@pytest.mark.parametrize("n,s,m,b", [(32, 1, 1, 5), (64, 3, 2, 4), (128, 4, 3, 3)])
+def test_analytical_update_matches_oracle(cuda, n, s, m, b):
+    """The reference's autograd=False update (Fourier projection + slice replay)."""
+    print(f"analytical n={n} s={s} m={m}", flush=True)
+    case = _case_with_targets(N=n, S=s, M=m, B=b, seed=300 + n, descan=False)
+    dev = cuda
+    props = _t(case["propagators"], dev) if s > 1 else None
+    loss, og, pg = qp.analytical_gradients(_t(case["obj"], dev), _t(case["probe"], dev), _t(case["positions"], dev),
+                                           props, _t(case["targets"], dev), _t(case["mask"], dev),
+                                           num_gpts=case["num_gpts"], mean_intensity=case["mean_intensity"])
+    ref_o, ref_p = po.analytical_update(case["obj"], case["probe"], case["positions"],
+                                        case["propagators"] if s > 1 else None, case["targets"])
+    assert rel(og.cpu().numpy(), ref_o) < GRAD_TOL
+    assert rel(pg.cpu().numpy(), ref_p) < GRAD_TOL

@@ -217,3 +217,34 @@ def test_golden_reference_recipe_fixture():
     assert abs(g["loss"] - float(z["loss"])) < 1e-9 * abs(float(z["loss"]))
     assert rel(g["obj"], z["grad_obj"]) < 1e-6
     assert rel(g["probe"], z["grad_probe"]) < 1e-6
+
+
+@pytest.mark.parametrize("M", [1, 3])
+def test_analytical_update_relates_to_autograd(M):
+    """The projection residual is -1/(2 scale) times the l2-amplitude gradient at the exit wave
+    (scale = P / (B mean_I), using a = sqrt(I + 1e-9) ~ |X|), and both replay the slices the same
+    way, so the reference's normalised analytical updates map back onto torch-autograd gradients.
+    Integer positions make the probe shift's adjoint the identity."""
+    case = make_case(N=16, S=3, M=M, B=4, seed=11, half_integer=False)
+    case["positions"] = np.round(case["positions"]).astype(np.float32)
+    I = po.forward(case["obj"], case["probe"], case["positions"], case["propagators"])["I"]
+    case = attach_targets(case, I)
+    og, pg = po.analytical_update(case["obj"], case["probe"], case["positions"], case["propagators"],
+                                  case["targets"])
+    ag = torch_ref.loss_and_grads(case["obj"], case["probe"], case["positions"], case["propagators"],
+                                  case["targets"], None, case["num_gpts"], case["mean_intensity"],
+                                  grad_positions=False)
+    scale = case["num_gpts"] / 4 / case["mean_intensity"]
+    fw = po.forward(case["obj"], case["probe"], case["positions"], case["propagators"])
+    rows, cols = fw["rows"], fw["cols"]
+    for s in range(3):
+        norm = 0.0
+        for m in range(M):
+            ov = np.zeros(case["obj"].shape[1:])
+            for b in range(4):
+                np.add.at(ov, (rows[b][:, None], cols[b][None, :]), np.abs(fw["psis"][s, m, b]) ** 2)
+            norm += ov.max()
+        assert rel(og[s] * (2 * scale * norm), ag["obj"][s]) < 1e-4
+    energy = np.sum(np.abs(fw["patches"]) ** 2, axis=(-2, -1)).max()
+    ortho = 2 * 16 if M == 1 else 1 / (2 * 16)
+    assert rel(pg * (2 * scale * energy * ortho), ag["probe"]) < 1e-4

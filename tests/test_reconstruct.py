@@ -73,3 +73,24 @@ def test_reconstruction_reduces_loss_and_error(cuda):
     est = rc.projected_phase(rec.obj_constrained.detach()).cpu().numpy()
     crop = ((40, 88), (40, 88))
     assert rc.phase_nmse(est, prob["true_phase"].sum(0), crop) < 0.7
+
+
+@pytest.mark.gpu
+def test_analytical_reconstruction_reduces_loss(cuda):
+    """autograd=False: the reference's projection update drives the loss down as well."""
+    spec = synth.ProblemSpec(name="single-slice-test", num_slices=1, obj_shape=(128, 128), roi=64, num_modes=1,
+                             mode_weights=(1.0,), scan=(12, 12), scan_step=4.0, jitter=0.5, dose=1e6, n_atoms=60,
+                             batch=32)
+    prob = synth.make_problem(spec, seed=1)
+    from quantem_amd.ptychography import forward_intensities
+
+    obj = torch.from_numpy(prob["obj"]).to(cuda)
+    probe = torch.from_numpy(prob["probe"]).to(cuda)
+    pos = torch.from_numpy(prob["positions"]).to(cuda)
+    I = forward_intensities(obj, probe, pos, None).cpu().numpy()
+    amps, mean_I = synth.poisson_amplitudes(I, np.random.default_rng(2))
+    rec = rc.Reconstruction(prob["positions"], amps, None, prob["probe"], (1,) + spec.obj_shape, mean_I,
+                            upsample_factor=spec.scan_step, device=cuda)
+    hist = rec.reconstruct(8, 32, obj_lr=2e-2, learn_probe=False, seed=0, autograd=False)
+    assert np.isfinite(hist).all()
+    assert hist[-1] < 0.8 * hist[0]
