@@ -74,13 +74,17 @@ def batches(P: int, B: int, seed: int) -> list:
     return [torch.from_numpy(perm[i:i + B].copy()) for i in range(0, P, B)]
 
 
-def gpu_pass(state: dict, data: dict, idx_list: list, scale: float) -> None:
+WORKLOADS = {"headline": synth.HEADLINE, "depth20-256px": synth.DEPTH}
+
+
+def gpu_pass(state: dict, data: dict, idx_list: list, scale: float, segment_slices: int = 0) -> None:
     obj_r, probe_r, opt = state["obj"], state["probe"], state["opt"]
     for idx in idx_list:
         opt.zero_grad(set_to_none=False)
         obj, probe = torch.view_as_complex(obj_r), torch.view_as_complex(probe_r)
         loss = qp.multislice_loss(obj, probe, data["positions"][idx], data["propagators"], data["targets"][idx],
-                                  None, None, num_gpts=data["num_gpts"], mean_intensity=data["mean_intensity"])
+                                  None, None, num_gpts=data["num_gpts"], mean_intensity=data["mean_intensity"],
+                                  segment_slices=segment_slices)
         loss.backward()
         obj_r.grad.mul_(scale)
         opt.step()
@@ -129,6 +133,12 @@ def main() -> int:
     ap.add_argument("--cpu-batch", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--workload", default="headline", choices=sorted(WORKLOADS),
+                    help="headline: 8 x 384^2, 128^2, 4 modes (the bench line); depth20-256px: 20 x 512^2, "
+                         "256^2, 2 modes (recomputation / peak-memory measurement)")
+    ap.add_argument("--segment", type=int, default=0,
+                    help="segment_slices of the backward pass: 0 stores every incident wave, k >= 2 keeps every "
+                         "k-th and recomputes the rest")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one elementwise kernel over 1 GiB first (torch.neg: 1 GiB read, 1 GiB written), "
                          "the reference point for the HBM byte counters")
@@ -142,7 +152,7 @@ def main() -> int:
         return 1
     dev = torch.device("cuda:0")
     _native.load()
-    spec = synth.HEADLINE
+    spec = WORKLOADS[args.workload]
     if args.calibrate_copy:
         src = torch.ones(1 << 28, dtype=torch.float32, device=dev)  # 1 GiB
         dst = torch.empty_like(src)
@@ -166,18 +176,20 @@ def main() -> int:
 
     log(f"warmup {args.warmup} steps")
     for _ in range(args.warmup):
-        gpu_pass(state, data, idx_list, scale)
+        gpu_pass(state, data, idx_list, scale, args.segment)
     torch.cuda.synchronize()
+    torch.cuda.reset_peak_memory_stats(dev)
     log(f"timing {args.steps} steps")
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     with _native.KernelTimer(max_launches=len(idx_list) * args.steps + 8) as kt:
         torch.cuda.synchronize()
         ev0.record()
         for _ in range(args.steps):
-            gpu_pass(state, data, idx_list, scale)
+            gpu_pass(state, data, idx_list, scale, args.segment)
         ev1.record()
         torch.cuda.synchronize()
     total_ms = ev0.elapsed_time(ev1)
+    peak_hbm = torch.cuda.max_memory_allocated(dev)
     ms_per_step = total_ms / args.steps
     value = P * args.steps / (total_ms / 1e3)
     log(f"{ms_per_step:.3f} ms/step, {value:.0f} patterns/s; fwd kernel {kt.fwd_ms / max(kt.n_fwd, 1):.3f} ms x "
@@ -189,10 +201,13 @@ def main() -> int:
     t_adj = kt.adj_ms / max(kt.n_adj, 1)
     dom, t_dom = ("multislice adjoint kernel", t_adj) if t_adj >= t_fwd else ("multislice forward kernel", t_fwd)
     achieved = flops_per_launch / (t_dom / 1e3) / 1e12
+    # measured counters exist for the headline workload's adjoint kernel with stored waves
+    measured = args.workload == "headline" and args.segment == 0 and dom == "multislice adjoint kernel"
+    traffic = ADJOINT_TRAFFIC_BYTES if measured else None
     roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-            "traffic": ADJOINT_TRAFFIC_BYTES if dom == "multislice adjoint kernel" else None,
-            "traffic_gbs": round(ADJOINT_TRAFFIC_BYTES / (t_dom / 1e3) / 1e9, 1) if dom == "multislice adjoint kernel" else None,
+            "traffic": traffic,
+            "traffic_gbs": round(traffic / (t_dom / 1e3) / 1e9, 1) if traffic else None,
             "kernel": dom, "kernel_ms": round(t_dom, 4),
             "fwd_kernel_ms": round(t_fwd, 4), "adj_kernel_ms": round(t_adj, 4),
             "note": "FP32 vector (FFT-flop) bound, no MFMA used; 157.3 TFLOP/s is the MI355X FP32 peak; "
@@ -216,7 +231,9 @@ def main() -> int:
         "dtype": "complex64",
         "data": "synthetic",
         "config": {"workload": spec.name, "slices": S, "object": [H, W], "roi": N, "modes": M,
-                   "scan": list(spec.scan), "patterns": P, "batch": B, "parallelism": "single-gpu"},
+                   "scan": list(spec.scan), "patterns": P, "batch": B, "parallelism": "single-gpu",
+                   "segment_slices": args.segment},
+        "peak_hbm_bytes": int(peak_hbm),
         "roofline": roof,
         "cpu_baseline": cpu,
     }

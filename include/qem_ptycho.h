@@ -65,13 +65,19 @@ typedef struct qem_ptycho_desc {
   int32_t num_slices;      /* S  >= 1 */
   int32_t num_modes;       /* M  >= 1 */
   int32_t batch;           /* B  >= 1, patterns in this call */
-  int32_t roi;             /* N  in {32, 64, 128} */
+  int32_t roi;             /* N  in {32, 64, 128, 256} */
   int32_t obj_rows;        /* Ho >= N */
   int32_t obj_cols;        /* Wo >= N */
   int32_t num_gpts;        /* P  total scan positions of the dataset (loss normalisation) */
   int32_t loss_type;       /* QEM_LOSS_* */
   float mean_intensity;    /* mean diffraction intensity of the dataset (> 0) */
   uint32_t flags;          /* QEM_FLAG_* */
+  int32_t segment_slices;  /* with QEM_FLAG_STORE_WAVES: 0, 1 or >= S keep the incident wave of every
+                              slice for the adjoint; k in [2, S) keeps only the wave entering every k-th
+                              slice and the adjoint recomputes the waves of one k-slice segment at a
+                              time (workspace B*M*(ceil(S/k) + k)*N*N*8 bytes instead of B*M*S*N*N*8,
+                              one extra forward propagation per slice).  No reference counterpart:
+                              torch autograd keeps every intermediate (ptychography_base.py:934-947). */
 } qem_ptycho_desc;
 
 /* Bytes of device workspace a forward_loss/adjoint pair needs for this descriptor.
@@ -137,7 +143,7 @@ int qem_profile_end(float* fwd_ms, int32_t* n_fwd, float* adj_ms, int32_t* n_adj
 /* Thread-local description of the last error (empty string if none). */
 const char* qem_last_error(void);
 
-/* ABI version (major * 100 + minor). */
+/* ABI version (major * 100 + minor): 101. */
 int32_t qem_abi_version(void);
 
 #ifdef __cplusplus

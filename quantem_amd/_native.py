@@ -65,6 +65,7 @@ class QemDesc(ctypes.Structure):
         ("loss_type", ctypes.c_int32),
         ("mean_intensity", ctypes.c_float),
         ("flags", ctypes.c_uint32),
+        ("segment_slices", ctypes.c_int32),
     ]
 
 

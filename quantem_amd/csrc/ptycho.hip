@@ -66,8 +66,13 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
   origin_and_shift<N>(p.pos, b, r0, c0, dr, dc);
 
   cf v[PTS];
-  // shifted probe: spectrum * ramp / N^2, inverse transform
-  {
+  if (p.fwd_mode == FWD_RECOMPUTE) {
+    // a segment of an earlier forward: start from its checkpoint wave
+    const cf* src = p.ckpt + ((size_t)item * p.nck + p.s0 / p.ck) * N * N;
+#pragma unroll
+    for (int e = 0; e < PTS; ++e) v[e] = src[f.p1_off(e)];
+  } else {
+    // shifted probe: spectrum * ramp / N^2, inverse transform
     const cf* pk = p.pk + (size_t)m * N * N;
     const cf rc = expm2pi(fftfreq1<N>(f.kc) * dc);
     const float inv_n2 = 1.0f / (float)(N * N);
@@ -77,10 +82,9 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
       const cf ramp = cmul(expm2pi(fftfreq1<N>(kr) * dr), rc);
       v[e] = cscale(cmul(pk[kr * N + f.kc], ramp), inv_n2);
     }
+    f.inv(v, lds);
   }
-  f.inv(v, lds);
 
-  // object rows / cols of this thread's elements (periodic wrap, fftfreq offsets)
   // byte offsets (32-bit, from a uniform slice base) of this thread's object rows / cols
   unsigned orow[F::RR1], ocol[F::RC1];
 #pragma unroll
@@ -94,19 +98,25 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
     ocol[i] = (unsigned)pmod(c0 + (c < N / 2 ? c : c - N), p.W) * 8u;
   }
 
-  const bool store = (p.flags & QEM_FLAG_STORE_WAVES) != 0;
+  const int mode = p.fwd_mode;
+  const bool far = mode != FWD_RECOMPUTE;
   const size_t HW = (size_t)p.H * p.W;
   const float inv_n2 = 1.0f / (float)(N * N);
-  for (int s = 0; s < p.S; ++s) {
+  for (int s = p.s0; s < p.s1; ++s) {
     const cf* O = p.obj + (size_t)s * HW;
-    if (store) {
-      cf* dst = p.psi + ((size_t)item * p.S + s) * N * N;
+    cf* dst = nullptr;
+    if (mode == FWD_STORE_ALL || mode == FWD_RECOMPUTE)
+      dst = p.psi + ((size_t)item * p.wslots + (s - p.s0)) * N * N;
+    else if (mode == FWD_CHECKPOINT && s % p.ck == 0)
+      dst = p.ckpt + ((size_t)item * p.nck + s / p.ck) * N * N;
+    if (dst != nullptr) {
 #pragma unroll
       for (int e = 0; e < PTS; ++e) dst[f.p1_off(e)] = v[e];
     }
+    if (!far && s + 1 == p.s1) break;  // a recomputed segment needs only the incident waves
 #pragma unroll
     for (int e = 0; e < PTS; ++e) v[e] = cmul(obj_at(O, orow[e / F::RC1] + ocol[e % F::RC1]), v[e]);
-    if (s + 1 < p.S) {
+    if (s + 1 < p.s1) {
       f.fwd(v, lds);
       const cf* pr = p.prop + (size_t)s * N * N;
 #pragma unroll
@@ -114,6 +124,7 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
       f.inv(v, lds);
     }
   }
+  if (!far) return;
   if constexpr (DESCAN) {
     const float ddr = p.descan[2 * b], ddc = p.descan[2 * b + 1];
 #pragma unroll
@@ -208,7 +219,12 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
 
   cf v[PTS];
   cf* Xs = p.X + (size_t)item * N * N;
-  if (p.flags & QEM_FLAG_ANALYTIC) {
+  const bool top = p.s1 == p.S;  // this launch starts at the far field
+  if (!top) {
+    // a lower segment: g_x of slice s1 left by the launch of the segment above (natural order)
+#pragma unroll
+    for (int e = 0; e < PTS; ++e) v[e] = Xs[f.p1_off(e)];
+  } else if (p.flags & QEM_FLAG_ANALYTIC) {
     // the projection kernel left the Fourier-space residual X' - X over X: g = ifft2_ortho(residual)
 #pragma unroll
     for (int e = 0; e < PTS; ++e) v[e] = cscale(Xs[f.p3_off(e)], inv_n);
@@ -221,7 +237,7 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
       v[e] = cscale(Xs[k], sc * gI[k]);
     }
   }
-  f.inv(v, lds);
+  if (top) f.inv(v, lds);
 
   // byte offsets (32-bit, from a uniform slice base) of this thread's object rows / cols
   unsigned orow[F::RR1], ocol[F::RC1];
@@ -240,9 +256,9 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
   const bool want_desc = DESCAN && (p.flags & QEM_FLAG_GRAD_DESCAN);
   // Descan (applied after the last slice in the forward): g_x <- conj(ramp) g_x, and its gradient
   // 2 pi sum f * Im(conj(g_x) x_out) with x_out = ramp * O_{S-1} psi_{S-1}.
-  if constexpr (DESCAN) {
+  if (DESCAN && top) {
     const cf* O = p.obj + (size_t)(p.S - 1) * HW;
-    const cf* ps = p.psi + ((size_t)item * p.S + p.S - 1) * N * N;
+    const cf* ps = p.psi + ((size_t)item * p.wslots + (p.S - 1 - p.s0)) * N * N;
     const float ddr = p.descan[2 * b], ddc = p.descan[2 * b + 1];
     double ar = 0.0, ac = 0.0;
 #pragma unroll
@@ -271,7 +287,7 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
   // stores), then g_psi = conj(O_s) g_x
   auto slice_step = [&](int s) {
     const cf* O = p.obj + (size_t)s * HW;
-    cf* ps = p.psi + ((size_t)item * p.S + s) * N * N;
+    cf* ps = p.psi + ((size_t)item * p.wslots + (s - p.s0)) * N * N;
     // chunks of 8 elements: the psi stores of one chunk keep the next chunk's loads from being
     // hoisted, which bounds the registers in flight
 #pragma unroll
@@ -289,7 +305,7 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
       }
     }
   };
-  for (int s = p.S - 1; s >= 0; --s) {
+  for (int s = p.s1 - 1; s >= p.s0; --s) {
     slice_step(s);
     if (s > 0) {
       // g_x <- ifft2(fft2(g_psi) conj(prop_{s-1})), the adjoint of the forward propagation
@@ -299,6 +315,12 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
       for (int e = 0; e < PTS; ++e) v[e] = cscale(cmulc(v[e], pr[f.p3_off(e)]), inv_n2);
       f.inv(v, lds);
     }
+  }
+  if (p.s0 > 0) {
+    // g_x of slice s0 - 1 for the segment below
+#pragma unroll
+    for (int e = 0; e < PTS; ++e) Xs[f.p1_off(e)] = v[e];
+    return;
   }
   f.fwd(v, lds);
   // G = fft2(g_psi_0), natural order, over X (the epilogue kernel turns it into the probe-gradient
@@ -403,7 +425,7 @@ __global__ __launch_bounds__(256) void k_overlap(Params p, float* out) {
       if (ux >= p.W) ux -= p.W;
       if (x >= p.W || ux >= N) continue;
       const int col = (ux + N / 2) & (N - 1);
-      const cf* g = p.psi + (((size_t)(b0 + i) * p.M + m) * p.S + s) * N * N + col;
+      const cf* g = p.psi + (((size_t)(b0 + i) * p.M + m) * p.wslots + s) * N * N + col;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int dy = ty + 8 * j;
@@ -449,14 +471,14 @@ __global__ __launch_bounds__(256) void k_patch_energy(Params p, float* out) {
 template <int N>
 __global__ __launch_bounds__(256) void k_obj_reduce(Params p) {
   __shared__ int su0[256], sv0[256];
-  const int s = blockIdx.z;
+  const int s = p.s0 + blockIdx.z;  // object slice; its patches are in psi slot blockIdx.z
   const int X0 = blockIdx.x * 32, Y0 = blockIdx.y * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
   const int x = X0 + tx;
   cf acc[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) acc[j] = mk(0.0f, 0.0f);
-  const size_t slab = (size_t)p.S * N * N;  // stride between modes of one pattern
+  const size_t slab = (size_t)p.wslots * N * N;  // stride between modes of one pattern
   for (int b0 = 0; b0 < p.B; b0 += 256) {
     const int nb = min(256, p.B - b0);
     if ((int)threadIdx.x < nb) {
@@ -475,7 +497,7 @@ __global__ __launch_bounds__(256) void k_obj_reduce(Params p) {
       if (ux >= p.W) ux -= p.W;
       const bool xok = x < p.W && ux < N;
       const int col = (ux + N / 2) & (N - 1);
-      const cf* g = p.psi + ((size_t)(b0 + i) * p.M * p.S + s) * N * N + col;
+      const cf* g = p.psi + ((size_t)(b0 + i) * p.M * p.wslots + blockIdx.z) * N * N + col;
       const cf* rowp[4];
       bool ok[4];
 #pragma unroll
@@ -661,22 +683,29 @@ void prof_mark(bool adjoint_kernel, int edge, hipStream_t st) {
 }
 
 struct Layout {
-  size_t psi, scratch, X, gI, loss_part, pk, gsum, gpos, gdesc, total;
+  size_t psi, ckpt, scratch, X, gI, loss_part, pk, gsum, gpos, gdesc, total;
   int nloss;
+  int wslots;  // wave slots per item in psi
+  int ck, nck; // checkpoint interval (0 = every incident wave is stored) and checkpoints per item
 };
 
 Layout layout_of(const qem_ptycho_desc& d) {
   Layout L;
   const size_t N2 = (size_t)d.roi * d.roi;
   const size_t BM = (size_t)d.batch * d.num_modes;
+  const int S = d.num_slices;
+  const bool store = (d.flags & QEM_FLAG_STORE_WAVES) != 0;
+  L.ck = (store && d.segment_slices >= 2 && d.segment_slices < S) ? d.segment_slices : 0;
+  L.nck = L.ck ? (S + L.ck - 1) / L.ck : 0;
+  // stored waves: all S slices, or one segment of ck slices (recomputed from a checkpoint)
+  L.wslots = store ? (L.ck ? L.ck : S) : 0;
   size_t o = 0;
   L.psi = o;
-  if (d.flags & QEM_FLAG_STORE_WAVES)
-    o += align_up(BM * d.num_slices * N2 * 8);
-  else if (d.roi == 256)
-    o += align_up(BM * N2 * 8);  // one rolling wave per item
+  o += align_up(BM * L.wslots * N2 * 8);
+  L.ckpt = o;
+  o += align_up(BM * L.nck * N2 * 8);
   L.scratch = o;
-  if (d.roi == 256) o += align_up(BM * 2 * N2 * 8);
+  if (d.roi == 256) o += align_up(BM * N2 * 8);  // the row-transformed wave of each item
   L.X = o;
   o += align_up(BM * N2 * 8);
   L.gI = o;
@@ -710,6 +739,7 @@ int check_desc(const qem_ptycho_desc* d) {
       (size_t)d->obj_rows * d->obj_cols > ((size_t)1 << 28))
     return fail(QEM_ERR_ARG, "object has too many elements (at most 2^28 per slice, 2^31 in total)");
   if (d->num_gpts < 1) return fail(QEM_ERR_ARG, "num_gpts must be >= 1 (got %d)", d->num_gpts);
+  if (d->segment_slices < 0) return fail(QEM_ERR_ARG, "segment_slices must be >= 0 (got %d)", d->segment_slices);
   if (d->loss_type < QEM_LOSS_L2_AMPLITUDE || d->loss_type > QEM_LOSS_POISSON)
     return fail(QEM_ERR_ARG, "unknown loss_type %d", d->loss_type);
   if (!(d->mean_intensity > 0.0f) || !isfinite(d->mean_intensity))
@@ -738,7 +768,15 @@ qem::Params make_params(const qem_ptycho_desc& d, const Layout& L, void* ws) {
   p.mean_I = d.mean_intensity;
   p.flags = d.flags;
   char* w = (char*)ws;
-  p.psi = ((d.flags & QEM_FLAG_STORE_WAVES) || d.roi == 256) ? (qem::cf*)(w + L.psi) : nullptr;
+  p.psi = L.wslots ? (qem::cf*)(w + L.psi) : nullptr;
+  p.ckpt = L.nck ? (qem::cf*)(w + L.ckpt) : nullptr;
+  p.wslots = L.wslots;
+  p.ck = L.ck;
+  p.nck = L.nck;
+  p.s0 = 0;
+  p.s1 = d.num_slices;
+  p.fwd_mode = !(d.flags & QEM_FLAG_STORE_WAVES) ? qem::FWD_NO_STORE
+                                                 : (L.ck ? qem::FWD_CHECKPOINT : qem::FWD_STORE_ALL);
   p.scratch = (d.roi == 256) ? (qem::cf*)(w + L.scratch) : nullptr;
   p.X = (qem::cf*)(w + L.X);
   p.gI = (float*)(w + L.gI);
@@ -780,13 +818,41 @@ int launch_forward256(const qem::Params& p, const Layout& L, hipStream_t st) {
 
 int launch_reductions(const qem::Params& p, hipStream_t st);
 
+// Slice segments of the adjoint, last first: [j ck, min(S, (j + 1) ck)) for j = nck-1 .. 0, or one
+// segment [0, S) when every incident wave is stored.
+template <typename Fn>
+int for_each_segment(const qem::Params& p, Fn&& fn) {
+  if (p.ck == 0) return fn(p);
+  for (int j = p.nck - 1; j >= 0; --j) {
+    qem::Params q = p;
+    q.s0 = j * p.ck;
+    q.s1 = (j + 1) * p.ck < p.S ? (j + 1) * p.ck : p.S;
+    q.fwd_mode = qem::FWD_RECOMPUTE;
+    if (int rc = fn(q)) return rc;
+  }
+  return QEM_OK;
+}
+
+template <int N>
+int launch_obj_reduce(const qem::Params& q, hipStream_t st) {
+  dim3 g((q.W + 31) / 32, (q.H + 31) / 32, q.s1 - q.s0);
+  hipLaunchKernelGGL(qem::k_obj_reduce<N>, g, dim3(256), 0, st, q);
+  return check_launch("obj_reduce");
+}
+
 int launch_adjoint256(const qem::Params& p, hipStream_t st) {
+  // with checkpoints, the timed region covers the recomputation and the per-segment object reduce
   prof_mark(true, 0, st);
-  if (int rc = qem::launch_chain256_adjoint(p, st)) return rc;
+  int rc = for_each_segment(p, [&](const qem::Params& q) {
+    if (q.fwd_mode == qem::FWD_RECOMPUTE)
+      if (int r = qem::launch_forward256_kernel(q, st)) return r;
+    if (int r = qem::launch_chain256_adjoint(q, st)) return r;
+    return q.ck ? launch_obj_reduce<256>(q, st) : QEM_OK;
+  });
   prof_mark(true, 1, st);
-  dim3 g((p.W + 31) / 32, (p.H + 31) / 32, p.S);
-  hipLaunchKernelGGL(qem::k_obj_reduce<256>, g, dim3(256), 0, st, p);
-  if (int rc = check_launch("obj_reduce")) return rc;
+  if (rc) return rc;
+  if (p.ck == 0)
+    if (int r = launch_obj_reduce<256>(p, st)) return r;
   hipLaunchKernelGGL(qem::k_probe_reduce<256>, dim3(p.M * 256 * 256 / 64), dim3(256), 0, st, p);
   if (int rc = check_launch("probe_reduce")) return rc;
   if (int rc = qem::launch_probe_grad256(p, st)) return rc;
@@ -810,18 +876,26 @@ int launch_reductions(const qem::Params& p, hipStream_t st) {
 template <int N>
 int launch_adjoint(const qem::Params& p, hipStream_t st) {
   using F = qem::Fft2<N>;
+  // with checkpoints, the timed region covers the recomputation and the per-segment object reduce
   prof_mark(true, 0, st);
-  if (p.descan != nullptr)
-    hipLaunchKernelGGL((qem::k_adjoint<N, true>), dim3(p.B * p.M), dim3(F::T), 0, st, p);
-  else
-    hipLaunchKernelGGL((qem::k_adjoint<N, false>), dim3(p.B * p.M), dim3(F::T), 0, st, p);
+  int rc = for_each_segment(p, [&](const qem::Params& q) {
+    if (q.fwd_mode == qem::FWD_RECOMPUTE) {
+      hipLaunchKernelGGL((qem::k_forward<N, false>), dim3(q.B * q.M), dim3(F::T), 0, st, q);
+      if (int r = check_launch("forward (recompute)")) return r;
+    }
+    if (q.descan != nullptr)
+      hipLaunchKernelGGL((qem::k_adjoint<N, true>), dim3(q.B * q.M), dim3(F::T), 0, st, q);
+    else
+      hipLaunchKernelGGL((qem::k_adjoint<N, false>), dim3(q.B * q.M), dim3(F::T), 0, st, q);
+    if (int r = check_launch("adjoint")) return r;
+    return q.ck ? launch_obj_reduce<N>(q, st) : QEM_OK;
+  });
   prof_mark(true, 1, st);
-  if (int rc = check_launch("adjoint")) return rc;
+  if (rc) return rc;
   hipLaunchKernelGGL(qem::k_adjoint_epilogue<N>, dim3(p.B * p.M), dim3(256), 0, st, p);
   if (int rc = check_launch("adjoint_epilogue")) return rc;
-  dim3 g((p.W + 31) / 32, (p.H + 31) / 32, p.S);
-  hipLaunchKernelGGL(qem::k_obj_reduce<N>, g, dim3(256), 0, st, p);
-  if (int rc = check_launch("obj_reduce")) return rc;
+  if (p.ck == 0)
+    if (int r = launch_obj_reduce<N>(p, st)) return r;
   hipLaunchKernelGGL(qem::k_probe_reduce<N>, dim3(p.M * N * N / 64), dim3(256), 0, st, p);
   if (int rc = check_launch("probe_reduce")) return rc;
   hipLaunchKernelGGL(qem::k_probe_grad<N>, dim3(p.M), dim3(F::T), 0, st, p);
@@ -868,7 +942,7 @@ int launch_projection_update(const qem::Params& p, const Layout& L, float* overl
 
 extern "C" {
 
-int32_t qem_abi_version(void) { return 100; }
+int32_t qem_abi_version(void) { return 101; }
 
 int qem_profile_begin(int32_t max_launches) {
   qem::clear_error();
@@ -1034,6 +1108,8 @@ int qem_ptycho_projection_update(const qem_ptycho_desc* desc, const void* obj, c
   if (!(d.flags & QEM_FLAG_STORE_WAVES))
     return fail(QEM_ERR_ARG, "projection update needs a forward pass run with QEM_FLAG_STORE_WAVES");
   if (d.roi == 256) return fail(QEM_ERR_UNSUPPORTED, "projection update is implemented for roi <= 128");
+  if (layout_of(d).ck != 0)
+    return fail(QEM_ERR_UNSUPPORTED, "projection update needs every incident wave stored (segment_slices 0)");
   if (int rc = check_ptr(obj, "obj", true)) return rc;
   if (int rc = check_ptr(probe, "probe", true)) return rc;
   if (int rc = check_ptr(positions, "positions", false)) return rc;

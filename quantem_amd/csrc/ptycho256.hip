@@ -1,12 +1,19 @@
-// ptycho256.hip — multislice forward / adjoint for 256 x 256 patterns (two-pass transforms).
+// ptycho256.hip — multislice forward / adjoint for 256 x 256 patterns (row / column passes).
 //
 // A 256 x 256 complex64 wave (512 KiB) does not fit one workgroup's LDS, so each workgroup owns one
-// (pattern, mode) item and streams every 2-D transform through two row passes: a pass reads 16 rows at
-// a time, runs length-256 FFTs (16 x 16, one LDS exchange), and writes the result transposed; two
-// passes give the 2-D transform in natural order.  The passes ping-pong between two per-item scratch
-// buffers (L2 / Infinity-Cache resident in practice), and the multislice elementwise steps
-// (transmission, propagator, descan, the adjoint's gradient patches) are fused into the passes' load
-// (pre) and store (post) operations.  The chain follows the same reference lines as ptycho.hip.
+// (pattern, mode) item and keeps its wave in a per-item buffer W, in the "row-transformed" domain
+// (r, kc): every row already Fourier transformed.  A slice then costs two passes over W:
+//
+//   row pass     for 16 rows at a time: inverse row DFT -> natural-order wave (r, c) -> the
+//                multislice elementwise work (store the incident wave, transmission, descan, the
+//                adjoint's gradient patch) -> forward row DFT -> back to W
+//   column pass  for 16 columns at a time: column DFT -> full 2-D spectrum (kr, kc) -> propagator
+//                (or far field / probe-gradient epilogue) -> inverse column DFT -> back to W
+//
+// A length-256 DFT is 16 x 16: a radix-16 DFT in registers (16 lanes per row or column, 16 values
+// per lane), a twiddle, an LDS exchange, and a second radix-16 DFT.  All passes update W in place
+// (every chunk is read completely before any of it is written) and end with a barrier.  The chain
+// follows the same reference lines as ptycho.hip.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -18,55 +25,100 @@ namespace n256 {
 
 constexpr int N = 256;
 constexpr int N2 = N * N;
-constexpr int LDX = 17;  // padded row stride (complex64) of the LDS buffers
+constexpr int T = 256;      // threads per workgroup (about 160 VGPRs: three workgroups per CU)
+constexpr int CH = T / 16;  // rows (or columns) per chunk: 16 lanes per row
+constexpr int VB = 272;     // LDS block per 16-lane vector: 16 x 17 (rows) or 16 x 16 + 16 (columns)
+static_assert(T == N, "one thread per frequency when tabulating the descan column factors");
 
 struct Smem {
-  cf ex[16 * 16 * LDX];  // per-row exchange of the 16 x 16 decomposition
-  cf tile[N * LDX];      // transposed output staging (N frequencies x 16 rows)
-  double red[4];
+  cf ex[CH * VB];  // exchange of the 16 x 16 decomposition
+  cf tw[N];        // W^k, W = exp(-2 pi i / 256)
+  cf dcol[N];      // column factor exp(-2 pi i fftfreq(c) d_c) of the descan ramp (adjoint)
+  double red[T / 64];
 };
 
-struct Ident {
-  __device__ __forceinline__ cf operator()(int, int, int, cf x) const { return x; }
-};
-struct IdentPost {
-  __device__ __forceinline__ cf operator()(int, int, cf x) const { return x; }
-};
+// Lane roles.  Row passes: vector (row) t / 16, j = t % 16.  Column passes: column t % 16 + 16 (t / 256)
+// (one 16-column group per 256 threads), j = (t / 16) % 16.  In both, the lane holds elements
+// n = j + 16 i of its vector in v[i].
+__device__ __forceinline__ int row_vec() { return threadIdx.x >> 4; }
+__device__ __forceinline__ int row_j() { return threadIdx.x & 15; }
+__device__ __forceinline__ int col_vec() { return (threadIdx.x & 15) + 16 * (threadIdx.x >> 8); }
+__device__ __forceinline__ int col_j() { return (threadIdx.x >> 4) & 15; }
 
-// out[k][r] = sum_c pre(r, c, i, src[r][c]) W^(+-c k) for every row r (W = exp(-2 pi i / 256), conjugated
-// if INV), stored as post(k, r, value).  c = j + 16 i with j = lane % 16 and i the unrolled slot.
-template <bool INV, typename Pre, typename Post>
-__device__ __forceinline__ void pass(const cf* __restrict__ src, cf* __restrict__ dst, Smem& sm, Pre&& pre,
-                                     Post&& post) {
-  const int t = threadIdx.x, rr = t >> 4, j = t & 15;
-  for (int r0 = 0; r0 < N; r0 += 16) {
-    const int r = r0 + rr;
+// Exchange slot of value (j, k1) of this lane's vector.  The row layout pads 16-value rows to 17, the
+// column layout pads each j block by 16, so that both the write (lanes vary j) and the read (lanes vary
+// k1) of a wave touch every bank at most twice (the minimum for 64 x 8 bytes).
+template <bool COLS>
+__device__ __forceinline__ int ex_idx(int j, int k1) {
+  if constexpr (COLS) {
+    const int cc = threadIdx.x & 15, grp = threadIdx.x >> 8;
+    return (grp * 16 + j) * VB + k1 * 16 + cc;
+  } else {
+    return row_vec() * VB + k1 * 17 + j;
+  }
+}
+
+// In-place length-256 DFT (unnormalised; conjugate twiddles if INV) of this lane's vector:
+// on entry v[i] = x[j + 16 i], on return v[k2] = X[j + 16 k2].
+template <bool INV, bool COLS>
+__device__ __forceinline__ void fft256(cf (&v)[16], Smem& sm) {
+  const int j = COLS ? col_j() : row_j();
+  Dft<16, INV>::template run<1>(v);  // over i -> k1
+#pragma unroll
+  for (int k1 = 1; k1 < 16; ++k1) {
+    const cf w = sm.tw[(j * k1) & 255];
+    v[k1] = INV ? cmulc(v[k1], w) : cmul(v[k1], w);
+  }
+  __syncthreads();  // earlier readers of sm.ex are done
+#pragma unroll
+  for (int k1 = 0; k1 < 16; ++k1) sm.ex[ex_idx<COLS>(j, k1)] = v[k1];
+  __syncthreads();
+#pragma unroll
+  for (int jj = 0; jj < 16; ++jj) v[jj] = sm.ex[ex_idx<COLS>(jj, j)];  // this lane now owns k1 = j
+  Dft<16, INV>::template run<1>(v);                                     // over jj -> k2
+}
+
+// Row pass over the 256 rows of src (CH rows per chunk): v <- row (columns j + 16 e); inverse row DFT
+// if PRE_INV; op(r, v) on the natural-order row; forward row DFT if POST_FWD; row -> dst (if not null).
+template <bool PRE_INV, bool POST_FWD, typename Op>
+__device__ __forceinline__ void row_pass(const cf* src, cf* dst, Smem& sm, Op&& op) {
+  const int j = row_j();
+  for (int r0 = 0; r0 < N; r0 += CH) {
+    const int r = r0 + row_vec();
     cf v[16];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) v[i] = pre(r, j + 16 * i, i, src[r * N + j + 16 * i]);
-    Dft<16, INV>::template run<1>(v);  // over i -> k1
+    for (int e = 0; e < 16; ++e) v[e] = src[r * N + j + 16 * e];
+    if constexpr (PRE_INV) fft256<true, false>(v, sm);
+    op(r, v);
+    if constexpr (POST_FWD) fft256<false, false>(v, sm);
+    if (dst != nullptr) {
 #pragma unroll
-    for (int k1 = 1; k1 < 16; ++k1) {
-      const cf w = kTw256[(j * k1) & 255];
-      v[k1] = INV ? cmulc(v[k1], w) : cmul(v[k1], w);
-    }
-    __syncthreads();  // the previous chunk is done with sm
-#pragma unroll
-    for (int k1 = 0; k1 < 16; ++k1) sm.ex[(rr * 16 + k1) * LDX + j] = v[k1];
-    __syncthreads();
-#pragma unroll
-    for (int jj = 0; jj < 16; ++jj) v[jj] = sm.ex[(rr * 16 + j) * LDX + jj];  // this lane now owns k1 = j
-    Dft<16, INV>::template run<1>(v);                                         // over jj -> k2; k = j + 16 k2
-#pragma unroll
-    for (int k2 = 0; k2 < 16; ++k2) sm.tile[(j + 16 * k2) * LDX + rr] = v[k2];
-    __syncthreads();
-#pragma unroll
-    for (int it = 0; it < 16; ++it) {
-      const int k = (t >> 4) + 16 * it, rw = t & 15;
-      dst[k * N + r0 + rw] = post(k, r0 + rw, sm.tile[k * LDX + rw]);
+      for (int e = 0; e < 16; ++e) dst[r * N + j + 16 * e] = v[e];
     }
   }
   __syncthreads();  // dst complete and visible to the whole workgroup
+}
+
+// Column pass over the 256 columns of src (CH columns per chunk): v <- column c (rows j + 16 e);
+// op(c, v) (which runs the column DFTs it needs); column -> dst.
+template <typename Op>
+__device__ __forceinline__ void col_pass(const cf* src, cf* dst, Smem& sm, Op&& op) {
+  const int j = col_j();
+  for (int c0 = 0; c0 < N; c0 += CH) {
+    const int c = c0 + col_vec();
+    cf v[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) v[e] = src[(j + 16 * e) * N + c];
+    op(c, v);
+#pragma unroll
+    for (int e = 0; e < 16; ++e) dst[(j + 16 * e) * N + c] = v[e];
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void init_tw(Smem& sm) {
+  if (threadIdx.x < N) sm.tw[threadIdx.x] = kTw256[threadIdx.x];
+  __syncthreads();
 }
 
 __device__ __forceinline__ cf ramp2(float fr, float fc, float dr, float dc) {
@@ -77,148 +129,250 @@ __device__ __forceinline__ cf ramp2(float fr, float fc, float dr, float dc) {
 __device__ __forceinline__ unsigned obj_row_off(int r0, int r, int H, int W) {
   return (unsigned)(pmod(r0 + (r < N / 2 ? r : r - N), H) * W) * 8u;
 }
-__device__ __forceinline__ unsigned obj_col_off(int c0, int c, int W) {
-  return (unsigned)pmod(c0 + (c < N / 2 ? c : c - N), W) * 8u;
+// Object columns of a row-pass lane: patch column j + 16 e is object column (q + 16 ((e + 8) % 16)) mod W
+// with q = (c0 - N/2 + j) mod W; since W >= N the wrap is one conditional subtraction.
+__device__ __forceinline__ int obj_col_base(int c0, int W) { return pmod(c0 - N / 2 + row_j(), W); }
+__device__ __forceinline__ unsigned obj_col_off(int q, int e, int W) {
+  int x = q + 16 * ((e + 8) & 15);
+  if (x >= W) x -= W;
+  return (unsigned)x * 8u;
 }
 
-__global__ __launch_bounds__(256) void k_spectrum256(Params p) {
+struct NoOp {
+  __device__ __forceinline__ void operator()(int, cf (&)[16]) const {}
+};
+
+// pk[m] = fft2(probe[m]).
+__global__ __launch_bounds__(T) void k_spectrum256(Params p) {
   __shared__ Smem sm;
+  init_tw(sm);
   const int m = blockIdx.x;
-  cf* b1 = p.scratch + (size_t)(2 * m) * N2;
-  pass<false>(p.probe + (size_t)m * N2, b1, sm, Ident(), IdentPost());
-  pass<false>(b1, p.pk + (size_t)m * N2, sm, Ident(), IdentPost());
+  cf* Wb = p.scratch + (size_t)m * N2;
+  row_pass<false, true>(p.probe + (size_t)m * N2, Wb, sm, NoOp());
+  col_pass(Wb, p.pk + (size_t)m * N2, sm, [&](int, cf(&v)[16]) { fft256<false, true>(v, sm); });
 }
 
+// Forward multislice of one (pattern, mode); the modes (p.fwd_mode) are those of k_forward<N>.
 template <bool DESCAN>
-__global__ __launch_bounds__(256) void k_forward256(Params p) {
+__global__ __launch_bounds__(T) void k_forward256(Params p) {
   __shared__ Smem sm;
+  init_tw(sm);
   const int item = xcd_item(blockIdx.x, gridDim.x);
   const int b = item / p.M, m = item - b * p.M;
   int r0, c0;
   float dr, dc;
   origin_and_shift<N>(p.pos, b, r0, c0, dr, dc);
-  cf* b1 = p.scratch + (size_t)(2 * item) * N2;
-  cf* b2 = b1 + N2;
-  const bool store = (p.flags & QEM_FLAG_STORE_WAVES) != 0;
-  auto slot = [&](int s) { return p.psi + ((size_t)item * (store ? p.S : 1) + (store ? s : 0)) * N2; };
+  cf* Wb = p.scratch + (size_t)item * N2;
+  const int mode = p.fwd_mode;
+  const bool far = mode != FWD_RECOMPUTE;
   const float inv_n2 = 1.0f / (float)N2;
   const size_t HW = (size_t)p.H * p.W;
-  unsigned ocol[16];
+  const int j = row_j();
+  const int q = obj_col_base(c0, p.W);
+  // where the incident wave of slice s is kept (nullptr: not kept)
+  auto slot = [&](int s) -> cf* {
+    if (mode == FWD_STORE_ALL || mode == FWD_RECOMPUTE) return p.psi + ((size_t)item * p.wslots + (s - p.s0)) * N2;
+    if (mode == FWD_CHECKPOINT && s % p.ck == 0) return p.ckpt + ((size_t)item * p.nck + s / p.ck) * N2;
+    return nullptr;
+  };
+
+  if (far) {
+    // psi_0 = ifft2(fft2(P_m) * ramp_b): inverse column DFT here, inverse row DFT in the first row pass
+    col_pass(p.pk + (size_t)m * N2, Wb, sm, [&](int kc, cf(&v)[16]) {
+      const float fc = fftfreq1<N>(kc);
+      const int jj = col_j();
 #pragma unroll
-  for (int i = 0; i < 16; ++i) ocol[i] = obj_col_off(c0, (threadIdx.x & 15) + 16 * i, p.W);
+      for (int e = 0; e < 16; ++e) v[e] = cscale(cmul(v[e], ramp2(fftfreq1<N>(jj + 16 * e), fc, dr, dc)), inv_n2);
+      fft256<true, true>(v, sm);
+    });
+  }
 
-  // psi_0 = ifft2(fft2(P_m) * ramp_b)
-  pass<true>(p.pk + (size_t)m * N2, b1, sm,
-             [&](int kr, int kc, int, cf x) { return cscale(cmul(x, ramp2(fftfreq1<N>(kr), fftfreq1<N>(kc), dr, dc)), inv_n2); },
-             IdentPost());
-  pass<true>(b1, slot(0), sm, Ident(), IdentPost());
-
-  for (int s = 0; s < p.S; ++s) {
+  for (int s = p.s0; s < p.s1; ++s) {
     const cf* O = p.obj + (size_t)s * HW;
-    const cf* A = slot(s);
-    auto transmit = [&](int r, int c, int i, cf x) {
-      cf y = cmul(obj_at(O, obj_row_off(r0, r, p.H, p.W) + ocol[i]), x);
-      if constexpr (DESCAN) y = cmul(y, ramp2(fftfreq1<N>(r), fftfreq1<N>(c), p.descan[2 * b], p.descan[2 * b + 1]));
-      return y;
+    cf* keep = slot(s);
+    const bool transmit = far || s + 1 < p.s1;  // a recomputed segment needs only its incident waves
+    const bool exit_ramp = DESCAN && far && s == p.S - 1;
+    auto op = [&](int r, cf(&v)[16]) {
+      if (keep != nullptr) {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) keep[r * N + j + 16 * e] = v[e];
+      }
+      if (transmit) {
+        const unsigned orow = obj_row_off(r0, r, p.H, p.W);
+#pragma unroll
+        for (int e = 0; e < 16; ++e) v[e] = cmul(obj_at(O, orow + obj_col_off(q, e, p.W)), v[e]);
+        if (exit_ramp) {
+          const float fr = fftfreq1<N>(r);
+#pragma unroll
+          for (int e = 0; e < 16; ++e)
+            v[e] = cmul(v[e], ramp2(fr, fftfreq1<N>(j + 16 * e), p.descan[2 * b], p.descan[2 * b + 1]));
+        }
+      }
     };
-    if (s + 1 < p.S) {
-      const cf* pr = p.prop + (size_t)s * N2;
-      pass<false>(A, b1, sm, [&](int r, int c, int i, cf x) { return cmul(obj_at(O, obj_row_off(r0, r, p.H, p.W) + ocol[i]), x); },
-                  IdentPost());
-      pass<false>(b1, b2, sm, Ident(), [&](int kr, int kc, cf v) { return cscale(cmul(v, pr[kr * N + kc]), inv_n2); });
-      pass<true>(b2, b1, sm, Ident(), IdentPost());
-      pass<true>(b1, slot(s + 1), sm, Ident(), IdentPost());
+    if (mode == FWD_RECOMPUTE && s == p.s0) {
+      // a segment of an earlier forward starts from its checkpoint wave (natural order)
+      const cf* ck = p.ckpt + ((size_t)item * p.nck + p.s0 / p.ck) * N2;
+      if (transmit)
+        row_pass<false, true>(ck, Wb, sm, op);
+      else
+        row_pass<false, false>(ck, nullptr, sm, op);
+    } else if (transmit) {
+      row_pass<true, true>(Wb, Wb, sm, op);
     } else {
+      row_pass<true, false>(Wb, nullptr, sm, op);
+    }
+    if (s + 1 < p.s1) {
+      const cf* pr = p.prop + (size_t)s * N2;
+      col_pass(Wb, Wb, sm, [&](int kc, cf(&v)[16]) {
+        fft256<false, true>(v, sm);
+        const int jj = col_j();
+#pragma unroll
+        for (int e = 0; e < 16; ++e) v[e] = cscale(cmul(v[e], pr[(jj + 16 * e) * N + kc]), inv_n2);
+        fft256<true, true>(v, sm);
+      });
+    } else if (far) {
       const float inv_n = 1.0f / (float)N;
-      pass<false>(A, b1, sm, transmit, IdentPost());
-      pass<false>(b1, p.X + (size_t)item * N2, sm, Ident(), [&](int, int, cf v) { return cscale(v, inv_n); });
+      col_pass(Wb, p.X + (size_t)item * N2, sm, [&](int, cf(&v)[16]) {
+        fft256<false, true>(v, sm);
+#pragma unroll
+        for (int e = 0; e < 16; ++e) v[e] = cscale(v[e], inv_n);
+      });
     }
   }
 }
 
+// Adjoint of one (pattern, mode) over the slice segment [s0, s1) (see k_adjoint<N>).  The top segment
+// starts from the far field; a lower one from the row-transformed g that the segment above left in X.
+// The bottom segment ends with G = fft2(g_psi_0): probe-gradient spectrum conj(ramp) G over X and the
+// position-gradient partial sums; any other segment leaves its row-transformed g in X.
 template <bool DESCAN>
-__global__ __launch_bounds__(256) void k_adjoint256(Params p) {
+__global__ __launch_bounds__(T) void k_adjoint256(Params p) {
   __shared__ Smem sm;
+  init_tw(sm);
   const int item = xcd_item(blockIdx.x, gridDim.x);
   const int b = item / p.M, m = item - b * p.M;
   int r0, c0;
   float dr, dc;
   origin_and_shift<N>(p.pos, b, r0, c0, dr, dc);
-  cf* b1 = p.scratch + (size_t)(2 * item) * N2;
-  cf* b2 = b1 + N2;
+  cf* Wb = p.scratch + (size_t)item * N2;
   cf* Xs = p.X + (size_t)item * N2;
-  const float* gI = p.gI + (size_t)b * N2;
-  const float dl = p.dloss[0];
   const float inv_n2 = 1.0f / (float)N2;
   const size_t HW = (size_t)p.H * p.W;
-  unsigned ocol[16];
-#pragma unroll
-  for (int i = 0; i < 16; ++i) ocol[i] = obj_col_off(c0, (threadIdx.x & 15) + 16 * i, p.W);
+  const int j = row_j();
+  const int q = obj_col_base(c0, p.W);
+  const bool top = p.s1 == p.S;
 
-  // g_x = (1/N) F^H (2 dL/dI X)
-  const float sc = 2.0f * dl / (float)N;
-  pass<true>(Xs, b1, sm, [&](int kr, int kc, int, cf x) { return cscale(x, sc * gI[kr * N + kc]); }, IdentPost());
-  pass<true>(b1, b2, sm, Ident(), IdentPost());
+  if (top) {
+    // g_x = (1/N) F^H (2 dL/dI X): inverse column DFT here, inverse row DFT in the first row pass
+    const float* gI = p.gI + (size_t)b * N2;
+    const float sc = 2.0f * p.dloss[0] / (float)N;
+    col_pass(Xs, Wb, sm, [&](int kc, cf(&v)[16]) {
+      const int jj = col_j();
+#pragma unroll
+      for (int e = 0; e < 16; ++e) v[e] = cscale(v[e], sc * gI[(jj + 16 * e) * N + kc]);
+      fft256<true, true>(v, sm);
+    });
+  }
 
   double dar = 0.0, dac = 0.0;
-  for (int s = p.S - 1; s >= 0; --s) {
+  if (DESCAN && top) {
+    // descan (applied after the last slice in the forward), in a pass of its own: its gradient
+    // 2 pi sum f Im(conj(g) x_out) with x_out = ramp O_{S-1} psi_{S-1}, and g <- conj(ramp) g
+    const cf* O = p.obj + (size_t)(p.S - 1) * HW;
+    const cf* A = p.psi + ((size_t)item * p.wslots + (p.S - 1 - p.s0)) * N2;
+    const float ddr = p.descan[2 * b];
+    sm.dcol[threadIdx.x] = expm2pi(fftfreq1<N>(threadIdx.x) * p.descan[2 * b + 1]);  // T == N
+    __syncthreads();
+    row_pass<true, true>(Wb, Wb, sm, [&](int r, cf(&v)[16]) {
+      const unsigned orow = obj_row_off(r0, r, p.H, p.W);
+      const float fr = fftfreq1<N>(r);
+      const cf drow = expm2pi(fr * ddr);
+      float sim = 0.0f, sfim = 0.0f;  // this row's sum of Im(...) and of fc Im(...)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const float fc = fftfreq1<N>(j + 16 * e);
+        const cf d = cmul(drow, sm.dcol[j + 16 * e]);
+        const cf xo = cmul(cmul(obj_at(O, orow + obj_col_off(q, e, p.W)), A[r * N + j + 16 * e]), d);
+        const float im = cconjmul(v[e], xo).y;
+        sim += im;
+        sfim += fc * im;
+        // 1/N: the forward row DFT below is not followed by a column pass, so it must leave R g / N
+        // (R = unnormalised row DFT), the form every row pass expects (R^H R = N)
+        v[e] = cscale(cmulc(v[e], d), 1.0f / (float)N);
+      }
+      dar += (double)(fr * sim);
+      dac += (double)sfim;
+    });
+  }
+  for (int s = p.s1 - 1; s >= p.s0; --s) {
     const cf* O = p.obj + (size_t)s * HW;
-    cf* A = p.psi + ((size_t)item * p.S + s) * N2;
-    const bool last = (s == p.S - 1);
-    // object-gradient patch conj(psi_s) g (over psi_s) and g_psi = conj(O_s) g, fused into the load
-    pass<false>(b2, b1, sm,
-                [&](int r, int c, int i, cf g) {
-                  const cf o = obj_at(O, obj_row_off(r0, r, p.H, p.W) + ocol[i]);
-                  const cf ps = A[r * N + c];
-                  if constexpr (DESCAN) {
-                    if (last) {
-                      const float fr = fftfreq1<N>(r), fc = fftfreq1<N>(c);
-                      const cf d = ramp2(fr, fc, p.descan[2 * b], p.descan[2 * b + 1]);
-                      const float im = cconjmul(g, cmul(cmul(o, ps), d)).y;
-                      dar += (double)(fr * im);
-                      dac += (double)(fc * im);
-                      g = cmulc(g, d);
-                    }
-                  }
-                  A[r * N + c] = cconjmul(ps, g);
-                  return cconjmul(o, g);
-                },
-                IdentPost());
+    cf* A = p.psi + ((size_t)item * p.wslots + (s - p.s0)) * N2;
+    // g_x of slice s in natural order: object-gradient patch conj(psi_s) g over psi_s, g_psi = conj(O_s) g
+    row_pass<true, true>((!top && s == p.s1 - 1) ? (const cf*)Xs : (const cf*)Wb, Wb, sm, [&](int r, cf(&v)[16]) {
+      const unsigned orow = obj_row_off(r0, r, p.H, p.W);
+      // two halves of 8 elements: all loads of a half before its stores
+#pragma unroll
+      for (int h = 0; h < 16; h += 8) {
+        cf ps[8], o[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          ps[i] = A[r * N + j + 16 * (h + i)];
+          o[i] = obj_at(O, orow + obj_col_off(q, h + i, p.W));
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const int e = h + i;
+          A[r * N + j + 16 * e] = cconjmul(ps[i], v[e]);
+          v[e] = cconjmul(o[i], v[e]);
+        }
+      }
+    });
     if (s > 0) {
+      // g_x <- ifft2(fft2(g_psi) conj(prop_{s-1})); at a segment bottom it goes to X for the segment below
       const cf* pr = p.prop + (size_t)(s - 1) * N2;
-      pass<false>(b1, b2, sm, Ident(), [&](int kr, int kc, cf v) { return cscale(cmulc(v, pr[kr * N + kc]), inv_n2); });
-      pass<true>(b2, b1, sm, Ident(), IdentPost());
-      pass<true>(b1, b2, sm, Ident(), IdentPost());
-    } else {
-      pass<false>(b1, b2, sm, Ident(), IdentPost());  // G = fft2(g_psi_0) in b2
+      col_pass(Wb, s == p.s0 ? Xs : Wb, sm, [&](int kc, cf(&v)[16]) {
+        fft256<false, true>(v, sm);
+        const int jj = col_j();
+#pragma unroll
+        for (int e = 0; e < 16; ++e) v[e] = cscale(cmulc(v[e], pr[(jj + 16 * e) * N + kc]), inv_n2);
+        fft256<true, true>(v, sm);
+      });
     }
   }
   if constexpr (DESCAN) {
-    if (p.flags & QEM_FLAG_GRAD_DESCAN) {
-      const double tr = block_sum<256>(dar, sm.red);
-      const double tc = block_sum<256>(dac, sm.red);
+    if (top && (p.flags & QEM_FLAG_GRAD_DESCAN)) {
+      const double tr = block_sum<T>(dar, sm.red);
+      const double tc = block_sum<T>(dac, sm.red);
       if (threadIdx.x == 0) {
         p.gdesc_part[2 * item] = (float)(2.0 * M_PI * tr);
         p.gdesc_part[2 * item + 1] = (float)(2.0 * M_PI * tc);
       }
     }
   }
-  // probe-gradient spectrum conj(ramp) G over X, position-gradient partial sums
+  if (p.s0 > 0) return;
+  // G = fft2(g_psi_0); probe-gradient spectrum conj(ramp_b) G over X, position partials
+  // (2 pi / N^2) sum f Im(conj(G) P_m ramp_b)
   const cf* pk = p.pk + (size_t)m * N2;
   double ar = 0.0, ac = 0.0;
-  for (int idx = threadIdx.x; idx < N2; idx += 256) {
-    const int kr = idx >> 8, kc = idx & (N - 1);
-    const float fr = fftfreq1<N>(kr), fc = fftfreq1<N>(kc);
-    const cf R = ramp2(fr, fc, dr, dc);
-    const cf G = b2[idx];
-    Xs[idx] = cmulc(G, R);
-    const float im = cconjmul(G, cmul(pk[idx], R)).y;
-    ar += (double)(fr * im);
-    ac += (double)(fc * im);
-  }
+  col_pass(Wb, Xs, sm, [&](int kc, cf(&v)[16]) {
+    fft256<false, true>(v, sm);
+    const int jj = col_j();
+    const float fc = fftfreq1<N>(kc);
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const int kr = jj + 16 * e;
+      const float fr = fftfreq1<N>(kr);
+      const cf R = ramp2(fr, fc, dr, dc);
+      const cf G = v[e];
+      const float im = cconjmul(G, cmul(pk[kr * N + kc], R)).y;
+      ar += (double)(fr * im);
+      ac += (double)(fc * im);
+      v[e] = cmulc(G, R);
+    }
+  });
   if (p.flags & QEM_FLAG_GRAD_POSITIONS) {
-    const double tr = block_sum<256>(ar, sm.red);
-    const double tc = block_sum<256>(ac, sm.red);
+    const double tr = block_sum<T>(ar, sm.red);
+    const double tc = block_sum<T>(ac, sm.red);
     if (threadIdx.x == 0) {
       p.gpos_part[2 * item] = (float)(2.0 * M_PI * tr * (double)inv_n2);
       p.gpos_part[2 * item + 1] = (float)(2.0 * M_PI * tc * (double)inv_n2);
@@ -226,37 +380,46 @@ __global__ __launch_bounds__(256) void k_adjoint256(Params p) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_probe_grad256(Params p) {
+// grad_probe[m] = ifft2(gsum[m]) (1/N^2 included).
+__global__ __launch_bounds__(T) void k_probe_grad256(Params p) {
   __shared__ Smem sm;
+  init_tw(sm);
   const int m = blockIdx.x;
-  cf* b1 = p.scratch + (size_t)(2 * m) * N2;
+  cf* Wb = p.scratch + (size_t)m * N2;
   const float inv_n2 = 1.0f / (float)N2;
-  pass<true>(p.gsum + (size_t)m * N2, b1, sm, Ident(), IdentPost());
-  pass<true>(b1, p.grad_probe + (size_t)m * N2, sm, Ident(), [&](int, int, cf v) { return cscale(v, inv_n2); });
+  col_pass(p.gsum + (size_t)m * N2, Wb, sm, [&](int, cf(&v)[16]) { fft256<true, true>(v, sm); });
+  row_pass<true, false>(Wb, p.grad_probe + (size_t)m * N2, sm, [&](int, cf(&v)[16]) {
+#pragma unroll
+    for (int e = 0; e < 16; ++e) v[e] = cscale(v[e], inv_n2);
+  });
 }
 
 }  // namespace n256
 
 int launch_chain256_forward(const Params& p, hipStream_t st) {
-  hipLaunchKernelGGL(n256::k_spectrum256, dim3(p.M), dim3(256), 0, st, p);
+  hipLaunchKernelGGL(n256::k_spectrum256, dim3(p.M), dim3(n256::T), 0, st, p);
   if (int rc = check_last_launch("spectrum256")) return rc;
-  if (p.descan != nullptr)
-    hipLaunchKernelGGL((n256::k_forward256<true>), dim3(p.B * p.M), dim3(256), 0, st, p);
+  return launch_forward256_kernel(p, st);
+}
+
+int launch_forward256_kernel(const Params& p, hipStream_t st) {
+  if (p.descan != nullptr && p.fwd_mode != FWD_RECOMPUTE)
+    hipLaunchKernelGGL((n256::k_forward256<true>), dim3(p.B * p.M), dim3(n256::T), 0, st, p);
   else
-    hipLaunchKernelGGL((n256::k_forward256<false>), dim3(p.B * p.M), dim3(256), 0, st, p);
+    hipLaunchKernelGGL((n256::k_forward256<false>), dim3(p.B * p.M), dim3(n256::T), 0, st, p);
   return check_last_launch("forward256");
 }
 
 int launch_chain256_adjoint(const Params& p, hipStream_t st) {
   if (p.descan != nullptr)
-    hipLaunchKernelGGL((n256::k_adjoint256<true>), dim3(p.B * p.M), dim3(256), 0, st, p);
+    hipLaunchKernelGGL((n256::k_adjoint256<true>), dim3(p.B * p.M), dim3(n256::T), 0, st, p);
   else
-    hipLaunchKernelGGL((n256::k_adjoint256<false>), dim3(p.B * p.M), dim3(256), 0, st, p);
+    hipLaunchKernelGGL((n256::k_adjoint256<false>), dim3(p.B * p.M), dim3(n256::T), 0, st, p);
   return check_last_launch("adjoint256");
 }
 
 int launch_probe_grad256(const Params& p, hipStream_t st) {
-  hipLaunchKernelGGL(n256::k_probe_grad256, dim3(p.M), dim3(256), 0, st, p);
+  hipLaunchKernelGGL(n256::k_probe_grad256, dim3(p.M), dim3(n256::T), 0, st, p);
   return check_last_launch("probe_grad256");
 }
 

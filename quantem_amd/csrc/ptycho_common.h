@@ -28,7 +28,7 @@ struct Params {
   float* grad_descan;
   // workspace
   cf* psi;          // (B*M*S, N, N)  incident waves; reused by the adjoint for per-mode object-gradient patches
-  cf* scratch;      // (B*M*2, N, N)  ping-pong buffers of the two-pass (N = 256) transforms
+  cf* scratch;      // (B*M, N, N)    row-transformed wave of each item (N = 256 chain)
   cf* X;            // (B*M, N, N)    far-field waves; reused by the adjoint for probe-gradient spectra
   float* gI;        // (B, N, N)      dL/dI, unshifted
   double* loss_part;  // per loss block
@@ -36,6 +36,23 @@ struct Params {
   cf* gsum;         // (M, N, N)      sum over the batch of probe-gradient spectra
   float* gpos_part;   // (B*M, 2)
   float* gdesc_part;  // (B*M, 2)
+  // slice range and wave storage of one launch.  psi holds `wslots` slots per item; slice s of the
+  // range lives in slot s - s0.  With checkpoints (ck >= 2) the full forward keeps only the wave
+  // entering every ck-th slice in ckpt (nck slots per item) and the adjoint runs segment by
+  // segment [s0, s1), recomputing the segment's waves from its checkpoint first.
+  int s0, s1;
+  int wslots;
+  int ck, nck;
+  int fwd_mode;  // FWD_* below
+  cf* ckpt;      // (B*M*nck, N, N)
+};
+
+// forward launch modes
+enum : int {
+  FWD_NO_STORE = 0,   // full chain, far field, no waves kept (no-grad forward)
+  FWD_STORE_ALL = 1,  // full chain, far field, every incident wave kept in psi
+  FWD_CHECKPOINT = 2, // full chain, far field, only the checkpoint waves kept in ckpt
+  FWD_RECOMPUTE = 3   // slices [s0, s1) from checkpoint s0 / ck into psi; no far field
 };
 
 __device__ __forceinline__ int pmod(int a, int n) {
@@ -94,7 +111,8 @@ __device__ __forceinline__ double block_sum(double x, double* red) {
 }
 
 // Two-pass (N = 256) chain launchers (ptycho256.hip).
-int launch_chain256_forward(const Params& p, hipStream_t st);
+int launch_chain256_forward(const Params& p, hipStream_t st);   // probe spectra + forward kernel
+int launch_forward256_kernel(const Params& p, hipStream_t st);  // forward kernel only (p.fwd_mode)
 int launch_chain256_adjoint(const Params& p, hipStream_t st);
 int launch_probe_grad256(const Params& p, hipStream_t st);
 

@@ -58,7 +58,7 @@ def _need(t: torch.Tensor, name: str, dtype: torch.dtype, shape: tuple, device: 
     return t.contiguous()
 
 
-def _desc(S, M, B, N, H, W, num_gpts, loss_type, mean_intensity, flags) -> nat.QemDesc:
+def _desc(S, M, B, N, H, W, num_gpts, loss_type, mean_intensity, flags, segment_slices=0) -> nat.QemDesc:
     if loss_type not in nat.LOSS_TYPES:
         raise ValueError(f"Unknown loss type {loss_type}, should be one of {sorted(nat.LOSS_TYPES)}")
     return nat.QemDesc(
@@ -72,6 +72,7 @@ def _desc(S, M, B, N, H, W, num_gpts, loss_type, mean_intensity, flags) -> nat.Q
         loss_type=nat.LOSS_TYPES[loss_type],
         mean_intensity=float(mean_intensity),
         flags=flags,
+        segment_slices=int(segment_slices),
     )
 
 
@@ -93,7 +94,7 @@ class MultisliceLoss(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, obj, probe, positions, propagators, targets, mask, descan, num_gpts, mean_intensity, loss_type,
-                return_intensities, grad_enabled):
+                return_intensities, grad_enabled, segment_slices):
         device = obj.device
         if device.type != "cuda":
             raise ValueError(f"obj must be on a GPU device, got {device}")
@@ -135,7 +136,7 @@ class MultisliceLoss(torch.autograd.Function):
                 flags |= nat.QEM_FLAG_GRAD_POSITIONS
             if desc_c is not None and ctx.needs_input_grad[6]:
                 flags |= nat.QEM_FLAG_GRAD_DESCAN
-        d = _desc(S, M, B, N, H, W, num_gpts, loss_type, mean_intensity, flags)
+        d = _desc(S, M, B, N, H, W, num_gpts, loss_type, mean_intensity, flags, segment_slices)
         ws = torch.empty(workspace_bytes(d), dtype=torch.uint8, device=device)
         loss = torch.empty((), dtype=torch.float32, device=device)
         intens = torch.empty((B, N, N), dtype=torch.float32, device=device) if return_intensities else None
@@ -169,7 +170,7 @@ class MultisliceLoss(torch.autograd.Function):
         nat.check(rc, "qem_ptycho_adjoint")
         return (g_obj if ctx.needs_input_grad[0] else None,
                 g_probe if ctx.needs_input_grad[1] else None,
-                g_pos, None, None, None, g_desc, None, None, None, None, None)
+                g_pos, None, None, None, g_desc, None, None, None, None, None, None)
 
 
 def multislice_loss(
@@ -185,6 +186,7 @@ def multislice_loss(
     mean_intensity: float,
     loss_type: str = "l2_amplitude",
     return_intensities: bool = False,
+    segment_slices: int = 0,
 ):
     """Batch loss of quantem's forward model (``ptychography.py:220-229``), differentiable.
 
@@ -201,14 +203,20 @@ def multislice_loss(
         mean_intensity: ``dset.mean_diffraction_intensity``.
         loss_type: one of l2_amplitude, l1_amplitude, l2_intensity, l1_intensity, poisson.
         return_intensities: also return the (B, N, N) fftshifted predicted intensities.
+        segment_slices: 0 keeps every slice's incident wave for the backward pass; k in [2, S) keeps
+            only every k-th one and recomputes the others in the backward pass, one k-slice segment
+            at a time (less memory, one extra propagation per slice).
 
     Returns:
         loss (0-d float32 tensor), and intensities if requested.
     """
     if loss_type not in nat.LOSS_TYPES:
         raise ValueError(f"Unknown loss type {loss_type}, should be 'l1' or 'l2'")
+    if int(segment_slices) < 0:
+        raise ValueError(f"segment_slices must be >= 0, got {segment_slices}")
     loss, intens = MultisliceLoss.apply(obj, probe, positions, propagators, targets, mask, descan, num_gpts,
-                                        mean_intensity, loss_type, return_intensities, torch.is_grad_enabled())
+                                        mean_intensity, loss_type, return_intensities, torch.is_grad_enabled(),
+                                        int(segment_slices))
     return (loss, intens) if return_intensities else loss
 
 

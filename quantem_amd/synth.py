@@ -169,6 +169,12 @@ class ProblemSpec:
 
 
 HEADLINE = ProblemSpec()
+# deep-adjoint workload: 20 slices of 512x512 at 1 A, 256x256 patterns, 2 modes (the reference's default
+# weights for two modes, probe_models.py:686-689), 48x48 raster; atom density as the headline slices
+DEPTH = ProblemSpec(
+    name="multislice-20x512-256px-2modes-48x48scan",
+    num_slices=20, obj_shape=(512, 512), roi=256, num_modes=2, mode_weights=(0.98, 0.02), scan=(48, 48),
+    slice_thickness=1.0, n_atoms=530, batch=256)
 SINGLE_SLICE = ProblemSpec(
     name="single-slice-192px-64px-32x32scan",
     num_slices=1, obj_shape=(192, 192), roi=64, num_modes=1, mode_weights=(1.0,), scan=(32, 32),

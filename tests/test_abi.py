@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
     lib = nat.load()
     for name in declared_functions():
         assert hasattr(lib, name), name
-    assert lib.qem_abi_version() == 100
+    assert lib.qem_abi_version() == 101
 
 
 def _desc(**kw):
@@ -51,7 +51,22 @@ def test_workspace_size_formula():
     assert ws - ws_nostore == 256 * 4 * 8 * 128 * 128 * 8
 
 
+def test_checkpoint_workspace_formula():
+    # segment_slices k in [2, S): ceil(S/k) checkpoints + one k-slice segment instead of S stored waves
+    full = qp.workspace_bytes(_desc(num_slices=20, roi=256, num_modes=2, batch=64, obj_rows=512, obj_cols=512))
+    ck = qp.workspace_bytes(_desc(num_slices=20, roi=256, num_modes=2, batch=64, obj_rows=512, obj_cols=512,
+                                  segment_slices=5))
+    wave = 64 * 2 * 256 * 256 * 8
+    assert full - ck == (20 - (4 + 5)) * wave
+    # 0, 1 and >= S all mean "store every slice"
+    for k in (0, 1, 20, 40):
+        assert qp.workspace_bytes(_desc(num_slices=20, segment_slices=k)) == qp.workspace_bytes(_desc(num_slices=20))
+    # without stored waves the setting is irrelevant
+    assert qp.workspace_bytes(_desc(flags=0, segment_slices=3)) == qp.workspace_bytes(_desc(flags=0))
+
+
 @pytest.mark.parametrize("field,value,code", [
+    ("segment_slices", -1, nat.QEM_ERR_ARG),
     ("roi", 96, nat.QEM_ERR_UNSUPPORTED), ("num_slices", 0, nat.QEM_ERR_ARG), ("batch", 0, nat.QEM_ERR_ARG),
     ("obj_rows", 64, nat.QEM_ERR_ARG), ("loss_type", 9, nat.QEM_ERR_ARG), ("mean_intensity", 0.0, nat.QEM_ERR_ARG),
     ("num_gpts", 0, nat.QEM_ERR_ARG), ("num_modes", 0, nat.QEM_ERR_ARG),
