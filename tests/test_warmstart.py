@@ -27,6 +27,31 @@ def test_atom_phase_fft_is_periodic_sum_of_gaussians():
     assert ph.sum() == pytest.approx(0.4 * 2 * np.pi * 1.5**2, rel=1e-6)  # integral of the Gaussian
 
 
+def test_features_oracle_on_known_patterns():
+    """Pins the feature restatement: a delta's CoM is its offset from (N/2, N/2) and all of its
+    intensity falls in the annulus holding its radius; a flat pattern has the grid's mean offset as
+    CoM (-1/2 for even N) and annular fractions equal to pixel counts; `amplitudes` squares the input
+    first."""
+    n = 32
+    edges = np.array([0.0, 2.0, 4.0, 6.0, 9.0, 12.0, 30.0])
+    pats = np.zeros((3, n, n))
+    pats[0, n // 2 + 3, n // 2 - 4] = 2.0   # radius 5 -> annulus [4, 6)
+    pats[1, n // 2, n // 2 + 10] = 1.0      # radius 10 -> annulus [9, 12)
+    pats[2] = 1.0
+    f = features_oracle.pattern_features(pats, edges, amplitudes=False)
+    assert f[0, :2] == pytest.approx([3.0, -4.0])
+    assert f[0, 2:] == pytest.approx([0, 0, 1, 0, 0, 0])
+    assert f[1, :2] == pytest.approx([0.0, 10.0])
+    assert f[1, 2:] == pytest.approx([0, 0, 0, 0, 1, 0])
+    r, k = np.meshgrid(np.arange(n) - n // 2, np.arange(n) - n // 2, indexing="ij")
+    rad = np.hypot(r, k)
+    counts = [np.count_nonzero((rad >= edges[a]) & (rad < edges[a + 1])) for a in range(6)]
+    assert f[2, :2] == pytest.approx([(np.arange(n) - n // 2).mean()] * 2)  # -0.5: even-size grid
+    assert f[2, 2:] == pytest.approx(np.array(counts) / n**2)
+    g = features_oracle.pattern_features(np.sqrt(pats), edges, amplitudes=True)
+    assert g == pytest.approx(f)
+
+
 
 @pytest.mark.gpu
 def test_features_match_oracle(cuda):
