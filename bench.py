@@ -80,7 +80,7 @@ WORKLOADS = {"headline": synth.HEADLINE, "depth20-256px": synth.DEPTH}
 def gpu_pass(state: dict, data: dict, idx_list: list, scale: float, segment_slices: int = 0) -> None:
     obj_r, probe_r, opt = state["obj"], state["probe"], state["opt"]
     for idx in idx_list:
-        opt.zero_grad(set_to_none=False)
+        opt.zero_grad(set_to_none=True)
         obj, probe = torch.view_as_complex(obj_r), torch.view_as_complex(probe_r)
         loss = qp.multislice_loss(obj, probe, data["positions"][idx], data["propagators"], data["targets"][idx],
                                   None, None, num_gpts=data["num_gpts"], mean_intensity=data["mean_intensity"],

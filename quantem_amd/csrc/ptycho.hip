@@ -111,7 +111,7 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
       dst = p.ckpt + ((size_t)item * p.nck + s / p.ck) * N * N;
     if (dst != nullptr) {
 #pragma unroll
-      for (int e = 0; e < PTS; ++e) dst[f.p1_off(e)] = v[e];
+      for (int e = 0; e < PTS; ++e) st_stream(dst + f.p1_off(e), v[e]);
     }
     if (!far && s + 1 == p.s1) break;  // a recomputed segment needs only the incident waves
 #pragma unroll
@@ -295,12 +295,12 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
       cf t[8], o[8];
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
-        t[i] = ps[f.p1_off(e0 + i)];
+        t[i] = ld_stream(ps + f.p1_off(e0 + i));
         o[i] = obj_at(O, orow[(e0 + i) / F::RC1] + ocol[(e0 + i) % F::RC1]);
       }
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
-        ps[f.p1_off(e0 + i)] = cconjmul(t[i], v[e0 + i]);
+        st_stream(ps + f.p1_off(e0 + i), cconjmul(t[i], v[e0 + i]));
         v[e0 + i] = cconjmul(o[i], v[e0 + i]);
       }
     }
@@ -516,7 +516,7 @@ __global__ __launch_bounds__(256) void k_obj_reduce(Params p) {
 #pragma unroll
           for (int mm = 0; mm < 4; ++mm) {
             const bool use = ok[j] && (m0 + mm < p.M);
-            t[j][mm] = *(use ? rowp[j] + (size_t)(m0 + mm) * slab : (const cf*)p.psi);
+            t[j][mm] = ld_stream(use ? rowp[j] + (size_t)(m0 + mm) * slab : (const cf*)p.psi);
             if (!use) t[j][mm] = mk(0.0f, 0.0f);
           }
         }

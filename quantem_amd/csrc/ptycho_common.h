@@ -90,6 +90,21 @@ __device__ __forceinline__ void origin_and_shift(const float* pos, int b, int& r
   dc = pc - rc;
 }
 
+// Streaming (nontemporal) access for the per-slice wave and gradient-patch buffers: each element is
+// written once and read once, long after, so it should not displace the object slices, propagators
+// and probe spectra from L2 / Infinity Cache.
+typedef float f2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void st_stream(cf* p, cf v) {
+  f2v x;
+  x.x = v.x;
+  x.y = v.y;
+  __builtin_nontemporal_store(x, (f2v*)p);
+}
+__device__ __forceinline__ cf ld_stream(const cf* p) {
+  const f2v x = __builtin_nontemporal_load((const f2v*)p);
+  return mk(x.x, x.y);
+}
+
 // Object element at a byte offset from the slice base.
 __device__ __forceinline__ cf obj_at(const cf* O, unsigned byte_off) {
   return *(const cf*)((const char*)O + byte_off);

@@ -165,7 +165,7 @@ class Reconstruction:
             total = torch.zeros((), dtype=torch.float32, device=self.device)
             for idx in batcher:
                 it = torch.from_numpy(idx).to(self.device)
-                opt.zero_grad(set_to_none=False)
+                opt.zero_grad(set_to_none=True)
                 if autograd:
                     loss = multislice_loss(self.obj_constrained, self.probe_constrained, self.positions[it],
                                            self.propagators, self.targets[it], self.mask, None,
