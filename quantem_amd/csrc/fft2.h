@@ -48,6 +48,17 @@ __device__ __forceinline__ cf cconjmul(cf a, cf b) { return mk(a.x * b.x + a.y *
 __device__ __forceinline__ cf cscale(cf a, float s) { return mk(a.x * s, a.y * s); }
 __device__ __forceinline__ float cabs2(cf a) { return a.x * a.x + a.y * a.y; }
 
+// Workgroup barrier that orders LDS only.  __syncthreads() is a workgroup-scope fence on all memory,
+// which makes every wave wait for its outstanding global stores before the barrier; the FFT exchanges
+// only hand data through LDS, so stores of stored waves / gradient patches issued before a transform
+// can drain while it runs.  Code that hands data between threads through global memory must keep
+// __syncthreads().
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Compile-time loop: f(std::integral_constant<int, I>) for I in [B, E).
 template <int B, int E, typename F>
 __device__ __forceinline__ void sfor(F&& f) {
@@ -313,13 +324,13 @@ struct Fft2 {
     sfor<0, RR1>([&](auto k1) {
       sfor<0, RC1>([&](auto kc0) { lds[e1p1 + A3 * k1.value + A1 * kc0.value] = v[k1.value * RC1 + kc0.value]; });
     });
-    __syncthreads();
+    lds_barrier();
     sfor<0, RR2>([&](auto r2) {
       sfor<0, RC2>([&](auto c0i) {
         v[r2.value * RC2 + c0i.value] = lds[e1p2 + A2 * RR3 * r2.value + c0i.value];
       });
     });
-    __syncthreads();
+    lds_barrier();
     // pass 2
     sfor<0, RR2>([&](auto r2) { Dft<RC2, false>::template run<1>(&v[r2.value * RC2]); });
     sfor<0, RC2>([&](auto kc1) { Dft<RR2, false>::template run<RC2>(&v[kc1.value]); });
@@ -331,7 +342,7 @@ struct Fft2 {
         lds[e2p2 + B2 * k2.value + RC1 * kc1.value] = v[k2.value * RC2 + kc1.value];
       });
     });
-    __syncthreads();
+    lds_barrier();
     sfor<0, H3>([&](auto h) {
       sfor<0, RR3>([&](auto r3) { v[h.value * RR3 + r3.value] = lds[e2p3[h.value] + B3 * r3.value]; });
     });
@@ -345,13 +356,13 @@ struct Fft2 {
     sfor<0, H3>([&](auto h) {
       sfor<0, RR3>([&](auto r3) { lds[e2p3[h.value] + B3 * r3.value] = v[h.value * RR3 + r3.value]; });
     });
-    __syncthreads();
+    lds_barrier();
     sfor<0, RR2>([&](auto k2) {
       sfor<0, RC2>([&](auto kc1) {
         v[k2.value * RC2 + kc1.value] = lds[e2p2 + B2 * k2.value + RC1 * kc1.value];
       });
     });
-    __syncthreads();
+    lds_barrier();
     sfor<1, RR2>([&](auto k2) {
       sfor<0, RC2>([&](auto kc1) { v[k2.value * RC2 + kc1.value] = cmulc(v[k2.value * RC2 + kc1.value], twl[twr_base + k2.value]); });
     });
@@ -362,7 +373,7 @@ struct Fft2 {
         lds[e1p2 + A2 * RR3 * r2.value + c0i.value] = v[r2.value * RC2 + c0i.value];
       });
     });
-    __syncthreads();
+    lds_barrier();
     sfor<0, RR1>([&](auto k1) {
       sfor<0, RC1>([&](auto kc0) { v[k1.value * RC1 + kc0.value] = lds[e1p1 + A3 * k1.value + A1 * kc0.value]; });
     });

@@ -285,21 +285,23 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
   }
   // one slice back: object-gradient patch conj(psi_s) g_x (written over psi_s, all loads before
   // stores), then g_psi = conj(O_s) g_x
+  static_assert(PTS % 16 == 0, "slice-step chunks of 16");
   auto slice_step = [&](int s) {
     const cf* O = p.obj + (size_t)s * HW;
     cf* ps = p.psi + ((size_t)item * p.wslots + (s - p.s0)) * N * N;
-    // chunks of 8 elements: the psi stores of one chunk keep the next chunk's loads from being
-    // hoisted, which bounds the registers in flight
+    // chunks of 16 elements: the psi stores of one chunk keep the next chunk's loads from being
+    // hoisted, which bounds the registers in flight; loads and stores share one vmcnt counter, so a
+    // chunk's loads also wait for the previous chunk's stores (fewer, larger chunks wait less)
 #pragma unroll
-    for (int e0 = 0; e0 < PTS; e0 += 8) {
-      cf t[8], o[8];
+    for (int e0 = 0; e0 < PTS; e0 += 16) {
+      cf t[16], o[16];
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
+      for (int i = 0; i < 16; ++i) {
         t[i] = ld_stream(ps + f.p1_off(e0 + i));
         o[i] = obj_at(O, orow[(e0 + i) / F::RC1] + ocol[(e0 + i) % F::RC1]);
       }
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
+      for (int i = 0; i < 16; ++i) {
         st_stream(ps + f.p1_off(e0 + i), cconjmul(t[i], v[e0 + i]));
         v[e0 + i] = cconjmul(o[i], v[e0 + i]);
       }
@@ -538,6 +540,88 @@ __global__ __launch_bounds__(256) void k_obj_reduce(Params p) {
     const int y = Y0 + ty + 8 * j;
     if (y < p.H) p.grad_obj[(size_t)s * p.H * p.W + (size_t)y * p.W + x] = acc[j];
   }
+}
+
+// Deterministic object-gradient scatter-add, one object row per block (the layout used when the row
+// fits the LDS): the block lists, in batch order, the patterns whose patch covers the row; each of
+// its N threads (one per stored patch column) then reads that patch row of every mode (N * 8
+// contiguous bytes per mode), sums the modes in order and adds the sum into a W-wide LDS row.  The
+// columns of one patch are distinct (W >= N), those of consecutive patches may coincide, so a barrier
+// separates the adds of consecutive patterns: every object pixel sums its patterns in increasing b.
+template <int N>
+__global__ __launch_bounds__(N) void k_obj_reduce_rows(Params p) {
+  extern __shared__ cf acc_row[];  // p.W entries
+  __shared__ int lb[N], lrow[N], lc0[N];
+  __shared__ int wcount[(N + 63) / 64];
+  constexpr int K = 4;  // patterns whose loads are in flight together
+  const int t = threadIdx.x;
+  const int y = blockIdx.x, slot = blockIdx.y, s = p.s0 + slot;
+  const int x_off = t < N / 2 ? t : t - N;  // natural column offset of stored patch column t
+  const size_t slab = (size_t)p.wslots * N * N;  // stride between modes of one pattern
+  for (int x = t; x < p.W; x += N) acc_row[x] = mk(0.0f, 0.0f);
+  for (int b0 = 0; b0 < p.B; b0 += N) {
+    // compact the patterns of this chunk that cover row y, keeping batch order
+    const int b = b0 + t;
+    int u = N;
+    int c0m = 0;
+    if (b < p.B) {
+      const int r0 = (int)rintf(p.pos[2 * b]), c0 = (int)rintf(p.pos[2 * b + 1]);
+      u = pmod(y - r0 + N / 2, p.H);
+      c0m = pmod(c0, p.W);
+    }
+    const bool hit = u < N;
+    const unsigned long long bal = __ballot(hit);
+    const int lane = t & 63, wave = t >> 6;
+    const int pre = __popcll(bal & ((1ull << lane) - 1ull));
+    if (lane == 0) wcount[wave] = __popcll(bal);
+    __syncthreads();  // also orders the zeroing / the previous chunk's reads of the list
+    int off = 0, cnt = 0;
+    for (int w = 0; w < (N + 63) / 64; ++w) {
+      off += w < wave ? wcount[w] : 0;
+      cnt += wcount[w];
+    }
+    if (hit) {
+      lb[off + pre] = b;
+      lrow[off + pre] = (u + N / 2) & (N - 1);
+      lc0[off + pre] = c0m;
+    }
+    __syncthreads();
+    for (int i0 = 0; i0 < cnt; i0 += K) {
+      cf val[K];
+      int xo[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        val[k] = mk(0.0f, 0.0f);
+        xo[k] = 0;
+        if (i0 + k < cnt) {
+          const int bb = lb[i0 + k];
+          const cf* g = p.psi + ((size_t)bb * p.M * p.wslots + slot) * N * N + (size_t)lrow[i0 + k] * N + t;
+          cf acc = ld_stream(g);
+          for (int m = 1; m < p.M; ++m) {
+            const cf v = ld_stream(g + (size_t)m * slab);
+            acc.x += v.x;
+            acc.y += v.y;
+          }
+          val[k] = acc;
+          int x = lc0[i0 + k] + x_off;  // in [-N/2, W + N/2): one wrap each way
+          x += x < 0 ? p.W : 0;
+          x -= x >= p.W ? p.W : 0;
+          xo[k] = x;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        if (i0 + k < cnt) {  // uniform over the block
+          acc_row[xo[k]].x += val[k].x;
+          acc_row[xo[k]].y += val[k].y;
+          __syncthreads();
+        }
+      }
+    }
+    __syncthreads();  // the list is rewritten by the next chunk
+  }
+  cf* out = p.grad_obj + ((size_t)s * p.H + y) * p.W;
+  for (int x = t; x < p.W; x += N) out[x] = acc_row[x];
 }
 
 // gsum[m][k] = sum_b spectra[b][m][k] in a fixed order: 64 frequencies per block, 4 lanes per
@@ -835,8 +919,13 @@ int for_each_segment(const qem::Params& p, Fn&& fn) {
 
 template <int N>
 int launch_obj_reduce(const qem::Params& q, hipStream_t st) {
-  dim3 g((q.W + 31) / 32, (q.H + 31) / 32, q.s1 - q.s0);
-  hipLaunchKernelGGL(qem::k_obj_reduce<N>, g, dim3(256), 0, st, q);
+  const size_t row_bytes = (size_t)q.W * sizeof(qem::cf);
+  if (row_bytes <= 48 * 1024) {
+    hipLaunchKernelGGL(qem::k_obj_reduce_rows<N>, dim3(q.H, q.s1 - q.s0), dim3(N), row_bytes, st, q);
+  } else {
+    dim3 g((q.W + 31) / 32, (q.H + 31) / 32, q.s1 - q.s0);
+    hipLaunchKernelGGL(qem::k_obj_reduce<N>, g, dim3(256), 0, st, q);
+  }
   return check_launch("obj_reduce");
 }
 
@@ -929,9 +1018,7 @@ int launch_projection_update(const qem::Params& p, const Layout& L, float* overl
   }
   hipLaunchKernelGGL((qem::k_adjoint<N, false>), dim3(p.B * p.M), dim3(F::T), 0, st, p);
   if (int rc = check_launch("adjoint")) return rc;
-  dim3 g((p.W + 31) / 32, (p.H + 31) / 32, p.S);
-  hipLaunchKernelGGL(qem::k_obj_reduce<N>, g, dim3(256), 0, st, p);
-  if (int rc = check_launch("obj_reduce")) return rc;
+  if (int rc = launch_obj_reduce<N>(p, st)) return rc;
   hipLaunchKernelGGL(qem::k_probe_reduce<N>, dim3(p.M * N * N / 64), dim3(256), 0, st, p);
   if (int rc = check_launch("probe_reduce")) return rc;
   hipLaunchKernelGGL(qem::k_probe_grad<N>, dim3(p.M), dim3(F::T), 0, st, p);

@@ -69,10 +69,10 @@ __device__ __forceinline__ void fft256(cf (&v)[16], Smem& sm) {
     const cf w = sm.tw[(j * k1) & 255];
     v[k1] = INV ? cmulc(v[k1], w) : cmul(v[k1], w);
   }
-  __syncthreads();  // earlier readers of sm.ex are done
+  lds_barrier();  // earlier readers of sm.ex are done
 #pragma unroll
   for (int k1 = 0; k1 < 16; ++k1) sm.ex[ex_idx<COLS>(j, k1)] = v[k1];
-  __syncthreads();
+  lds_barrier();
 #pragma unroll
   for (int jj = 0; jj < 16; ++jj) v[jj] = sm.ex[ex_idx<COLS>(jj, j)];  // this lane now owns k1 = j
   Dft<16, INV>::template run<1>(v);                                     // over jj -> k2
