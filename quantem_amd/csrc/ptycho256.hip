@@ -37,6 +37,15 @@ struct Smem {
   double red[T / 64];
 };
 
+// The multislice kernels run one workgroup per CU: with 256 items resident the row-transformed waves
+// (256 x 512 KiB) stay in the Infinity Cache across the two passes of every slice, which measured
+// faster than two or three items per CU streaming twice that through HBM.  An LDS footprint above
+// half a CU's 160 KiB enforces it.
+struct SmemOne : Smem {
+  char occupancy_pad[82 * 1024 - sizeof(Smem)];
+};
+static_assert(sizeof(SmemOne) > 80 * 1024, "one workgroup per CU");
+
 // Lane roles.  Row passes: vector (row) t / 16, j = t % 16.  Column passes: column t % 16 + 16 (t / 256)
 // (one 16-column group per 256 threads), j = (t / 16) % 16.  In both, the lane holds elements
 // n = j + 16 i of its vector in v[i].
@@ -154,8 +163,8 @@ __global__ __launch_bounds__(T) void k_spectrum256(Params p) {
 
 // Forward multislice of one (pattern, mode); the modes (p.fwd_mode) are those of k_forward<N>.
 template <bool DESCAN>
-__global__ __launch_bounds__(T) void k_forward256(Params p) {
-  __shared__ Smem sm;
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_forward256(Params p) {
+  __shared__ SmemOne sm;
   init_tw(sm);
   const int item = xcd_item(blockIdx.x, gridDim.x);
   const int b = item / p.M, m = item - b * p.M;
@@ -195,7 +204,7 @@ __global__ __launch_bounds__(T) void k_forward256(Params p) {
     auto op = [&](int r, cf(&v)[16]) {
       if (keep != nullptr) {
 #pragma unroll
-        for (int e = 0; e < 16; ++e) keep[r * N + j + 16 * e] = v[e];
+        for (int e = 0; e < 16; ++e) st_stream(keep + r * N + j + 16 * e, v[e]);
       }
       if (transmit) {
         const unsigned orow = obj_row_off(r0, r, p.H, p.W);
@@ -246,8 +255,8 @@ __global__ __launch_bounds__(T) void k_forward256(Params p) {
 // The bottom segment ends with G = fft2(g_psi_0): probe-gradient spectrum conj(ramp) G over X and the
 // position-gradient partial sums; any other segment leaves its row-transformed g in X.
 template <bool DESCAN>
-__global__ __launch_bounds__(T) void k_adjoint256(Params p) {
-  __shared__ Smem sm;
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_adjoint256(Params p) {
+  __shared__ SmemOne sm;
   init_tw(sm);
   const int item = xcd_item(blockIdx.x, gridDim.x);
   const int b = item / p.M, m = item - b * p.M;
@@ -316,13 +325,13 @@ __global__ __launch_bounds__(T) void k_adjoint256(Params p) {
         cf ps[8], o[8];
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-          ps[i] = A[r * N + j + 16 * (h + i)];
+          ps[i] = ld_stream(A + r * N + j + 16 * (h + i));
           o[i] = obj_at(O, orow + obj_col_off(q, h + i, p.W));
         }
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
           const int e = h + i;
-          A[r * N + j + 16 * e] = cconjmul(ps[i], v[e]);
+          st_stream(A + r * N + j + 16 * e, cconjmul(ps[i], v[e]));
           v[e] = cconjmul(o[i], v[e]);
         }
       }
