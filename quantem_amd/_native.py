@@ -50,6 +50,7 @@ EXPORTED_SYMBOLS = (
     "qem_conv3x3_forward",
     "qem_conv3x3_wgrad_workspace_bytes",
     "qem_conv3x3_wgrad",
+    "qem_adam_step_f32",
 )
 
 
@@ -104,6 +105,8 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.qem_conv3x3_wgrad_workspace_bytes.restype = ctypes.c_int
     lib.qem_conv3x3_wgrad.argtypes = [i32] * 5 + [_vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]
     lib.qem_conv3x3_wgrad.restype = ctypes.c_int
+    lib.qem_adam_step_f32.argtypes = [ctypes.c_int64, _vp, _vp, _vp, _vp] + [ctypes.c_double] * 6 + [i32, _vp]
+    lib.qem_adam_step_f32.restype = ctypes.c_int
     lib.qem_abi_version.argtypes = []
     lib.qem_abi_version.restype = ctypes.c_int32
 

@@ -6,7 +6,8 @@ probe on the autograd graph (``object_models.py:289-339``, ``probe_models.py:452
 and its gradient (here one call of :func:`quantem_amd.ptychography.multislice_loss`), the object
 gradient rescale by ``upsample_factor**2 / 2`` (``ptychography.py:330-332``) and an Adam step with the
 reference's default learning rates (object 5e-3, probe 1e-3; ``object_models.py:44-48``,
-``probe_models.py:52-54``).  The object starts flat (``ObjectPixelated.from_uniform``,
+``probe_models.py:52-54``); the rescale and the Adam step run as one kernel per parameter
+(:class:`quantem_amd.optim.Adam`).  The object starts flat (``ObjectPixelated.from_uniform``,
 ``object_models.py:549-571, 653-657``) unless an initial object is given (the warm start).
 """
 
@@ -19,6 +20,7 @@ from typing import Callable, Optional
 import numpy as np
 import torch
 
+from .optim import Adam
 from .ptychography import analytical_gradients, multislice_loss
 
 
@@ -118,8 +120,7 @@ class Reconstruction:
                 tuple(int(x) for x in obj_shape), dtype=odt, device=dev)
         else:
             obj0 = torch.as_tensor(obj_init, dtype=odt, device=dev).clone()
-        # complex parameters live in real (..., 2) storage so that the fused Adam kernel applies
-        # (it is the complex Adam's update, which works on the same real view)
+        # complex parameters live in real (..., 2) storage, the view the optimizer updates
         self._obj = torch.nn.Parameter(_real_view(obj0).clone())
         self._probe = torch.nn.Parameter(
             _real_view(torch.as_tensor(probe, dtype=torch.complex64, device=dev)).clone())
@@ -152,13 +153,15 @@ class Reconstruction:
         replay, ``ptychography.py:339-372``) instead of the loss gradient; as in the reference, its
         gradients are assigned to the raw parameters without the upsample rescale.
         """
-        groups = [{"params": [self._obj], "lr": obj_lr}]
+        # the object-gradient rescale of the autograd path (ptychography.py:330-332) is applied inside
+        # the optimizer step; the analytical update is not rescaled, as in the reference
+        scale = self.upsample_factor ** 2 / 2 if autograd else 1.0
+        groups = [{"params": [self._obj], "lr": obj_lr, "grad_scale": scale}]
         if learn_probe:
             groups.append({"params": [self._probe], "lr": probe_lr})
         self._probe.requires_grad_(learn_probe)
-        opt = torch.optim.Adam(groups, fused=self.device.type == "cuda")
+        opt = Adam(groups)
         batcher = SimpleBatcher(self.num_gpts, batch_size, rng=seed)
-        scale = self.upsample_factor ** 2 / 2
         t0 = time.perf_counter()
         t_cb = 0.0  # time spent in callbacks, excluded from self.times
         for epoch in range(num_iters):
@@ -172,7 +175,6 @@ class Reconstruction:
                                            num_gpts=self.num_gpts, mean_intensity=self.mean_intensity,
                                            loss_type=loss_type)
                     loss.backward()
-                    self._obj.grad.mul_(scale)
                 else:
                     with torch.no_grad():
                         loss, og, pg = analytical_gradients(

@@ -238,3 +238,16 @@ def test_analytical_update_matches_oracle(cuda, n, s, m, b):
                                         case["propagators"] if s > 1 else None, case["targets"])
     assert rel(og.cpu().numpy(), ref_o) < GRAD_TOL
     assert rel(pg.cpu().numpy(), ref_p) < GRAD_TOL
+
+
+@pytest.mark.parametrize("n,s,m,b,h,w", [(32, 2, 2, 70, 56, 48), (64, 2, 2, 130, 96, 80), (32, 2, 2, 5, 40, 6400)])
+def test_object_reduce_chunks_and_wide_rows(cuda, n, s, m, b, h, w):
+    """Batches larger than the row reduce's N-pattern chunks, and rows wider than its LDS row
+    (6400 columns: the 32 x 32 tile reduce)."""
+    print(f"reduce n={n} b={b} obj=({h},{w})", flush=True)
+    case = _case_with_targets(N=n, S=s, M=m, B=b, H=h, W=w, seed=700 + b)
+    got = _run(case, cuda)
+    ref = _oracle(case)
+    assert rel(got["I"], ref["I"]) < FWD_TOL
+    for k in ("obj", "probe", "positions"):
+        assert rel(got[k], ref[k]) < GRAD_TOL, k

@@ -94,3 +94,39 @@ def test_analytical_reconstruction_reduces_loss(cuda):
     hist = rec.reconstruct(8, 32, obj_lr=2e-2, learn_probe=False, seed=0, autograd=False)
     assert np.isfinite(hist).all()
     assert hist[-1] < 0.8 * hist[0]
+
+
+@pytest.mark.gpu
+def test_single_slice_reference_case(cuda):
+    """The reference's CPU-runnable case end to end on the GPU: 192x192 pure-phase object (300 Gaussian
+    atoms), 64x64 probe (20 mrad, 300 kV, 0.1 A/px, 100 A defocus), 32x32 raster at 4 px with +-0.5 px
+    jitter, Poisson 1e4 e-/pattern, 20 epochs of Adam at the reference learning rates, batch 64, flat
+    start.  Prints the wall-clock; the loss and the projected-phase error over the scanned window fall."""
+    import time
+
+    from quantem_amd.ptychography import forward_intensities
+
+    spec = synth.SINGLE_SLICE
+    prob = synth.make_problem(spec, seed=0)
+    I = forward_intensities(torch.from_numpy(prob["obj"]).to(cuda), torch.from_numpy(prob["probe"]).to(cuda),
+                            torch.from_numpy(prob["positions"]).to(cuda), None).cpu().numpy()
+    amps, mean_I = synth.poisson_amplitudes(I, np.random.default_rng(1))
+    rec = rc.Reconstruction(prob["positions"], amps, None, prob["probe"], (1,) + spec.obj_shape, mean_I,
+                            upsample_factor=spec.scan_step, device=cuda)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = rec.reconstruct(20, spec.batch, seed=0)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    o = synth.scan_origin(spec)
+    ext = (np.asarray(spec.scan) - 1) * spec.scan_step
+    crop = tuple((int(np.floor(o[i])), int(np.ceil(o[i] + ext[i])) + 1) for i in range(2))
+    est = rc.projected_phase(rec.obj_constrained.detach()).cpu().numpy()
+    nmse = rc.phase_nmse(est, prob["true_phase"].sum(0), crop)
+    print(f"single-slice reference case: 20 epochs x {amps.shape[0]} patterns in {dt:.2f} s, "
+          f"loss {hist[0]:.4g} -> {hist[-1]:.4g}, phase NMSE over the scan {nmse:.3f} (flat start 1.0)", flush=True)
+    assert np.isfinite(hist).all()
+    # at 1e4 e-/pattern the Poisson noise sets a high loss floor (measured 7.32 -> 6.62); the phase
+    # error is the informative check (measured 0.238)
+    assert hist[-1] < 0.95 * hist[0]
+    assert nmse < 0.4
