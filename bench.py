@@ -75,8 +75,6 @@ def batches(P: int, B: int, seed: int) -> list:
     return [torch.from_numpy(perm[i:i + B].copy()) for i in range(0, P, B)]
 
 
-WORKLOADS = {"headline": synth.HEADLINE, "depth20-256px": synth.DEPTH}
-
 
 def gpu_pass(state: dict, data: dict, idx_list: list, segment_slices: int = 0) -> None:
     obj, probe, opt = state["obj"], state["probe"], state["opt"]
@@ -132,9 +130,10 @@ def main() -> int:
     ap.add_argument("--cpu-batch", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--workload", default="headline", choices=sorted(WORKLOADS),
+    ap.add_argument("--workload", default="headline", choices=sorted(synth.WORKLOADS),
                     help="headline: 8 x 384^2, 128^2, 4 modes (the bench line); depth20-256px: 20 x 512^2, "
-                         "256^2, 2 modes (recomputation / peak-memory measurement)")
+                         "256^2, 2 modes; large30-1024px: 30 x 1024^2, 256^2, 8 modes, 16384 patterns; "
+                         "single-slice: the reference's CPU case")
     ap.add_argument("--segment", type=int, default=0,
                     help="segment_slices of the backward pass: 0 stores every incident wave, k >= 2 keeps every "
                          "k-th and recomputes the rest")
@@ -151,7 +150,7 @@ def main() -> int:
         return 1
     dev = torch.device("cuda:0")
     _native.load()
-    spec = WORKLOADS[args.workload]
+    spec = synth.WORKLOADS[args.workload]
     if args.calibrate_copy:
         src = torch.ones(1 << 28, dtype=torch.float32, device=dev)  # 1 GiB
         dst = torch.empty_like(src)

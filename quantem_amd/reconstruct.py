@@ -145,13 +145,15 @@ class Reconstruction:
 
     def reconstruct(self, num_iters: int, batch_size: int, obj_lr: float = 5e-3, probe_lr: float = 1e-3,
                     learn_probe: bool = True, loss_type: str = "l2_amplitude", seed: int = 0,
-                    autograd: bool = True,
+                    autograd: bool = True, segment_slices: int = 0,
                     callback: Optional[Callable[[int, float, "Reconstruction"], None]] = None) -> list:
         """Runs `num_iters` epochs; returns the per-epoch mean batch loss.
 
         ``autograd=False`` selects the reference's analytical update (Fourier projection and slice
         replay, ``ptychography.py:339-372``) instead of the loss gradient; as in the reference, its
         gradients are assigned to the raw parameters without the upsample rescale.
+        ``segment_slices`` k >= 2 keeps every k-th incident wave and recomputes the rest in the
+        backward pass (less memory; see :func:`quantem_amd.ptychography.multislice_loss`).
         """
         # the object-gradient rescale of the autograd path (ptychography.py:330-332) is applied inside
         # the optimizer step; the analytical update is not rescaled, as in the reference
@@ -173,7 +175,7 @@ class Reconstruction:
                     loss = multislice_loss(self.obj_constrained, self.probe_constrained, self.positions[it],
                                            self.propagators, self.targets[it], self.mask, None,
                                            num_gpts=self.num_gpts, mean_intensity=self.mean_intensity,
-                                           loss_type=loss_type)
+                                           loss_type=loss_type, segment_slices=segment_slices)
                     loss.backward()
                 else:
                     with torch.no_grad():
@@ -194,3 +196,99 @@ class Reconstruction:
                 callback(epoch, mean_loss, self)
                 t_cb += time.perf_counter() - tc
         return self.history
+
+
+def scan_window(spec) -> tuple:
+    """Rows / columns of the object covered by the scan positions (the NMSE crop)."""
+    from .synth import scan_origin
+
+    o = scan_origin(spec)
+    ext = (np.asarray(spec.scan) - 1) * spec.scan_step
+    return tuple((int(np.floor(o[i])), int(np.ceil(o[i] + ext[i])) + 1) for i in range(2))
+
+
+def simulate_amplitudes(spec, prob: dict, device, seed: int = 0) -> tuple:
+    """Measured amplitudes on the GPU: the forward kernel's intensities (batches of spec.batch) with
+    Poisson counts at the probe's dose; returns (amplitudes (P, N, N) float32 on device, mean intensity)."""
+    from .ptychography import forward_intensities
+
+    dev = torch.device(device)
+    obj = torch.from_numpy(prob["obj"]).to(dev)
+    probe = torch.from_numpy(prob["probe"]).to(dev)
+    pos = torch.from_numpy(prob["positions"]).to(dev)
+    props = torch.from_numpy(prob["propagators"]).to(dev) if spec.num_slices > 1 else None
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed + 1)
+    P = pos.shape[0]
+    amps = torch.empty((P, spec.roi, spec.roi), dtype=torch.float32, device=dev)
+    total = torch.zeros((), dtype=torch.float64, device=dev)
+    for i in range(0, P, spec.batch):
+        counts = torch.poisson(forward_intensities(obj, probe, pos[i:i + spec.batch], props).clamp_min(0),
+                               generator=gen)
+        total += counts.sum(dtype=torch.float64)
+        amps[i:i + spec.batch] = torch.sqrt(counts)
+    return amps, float(total.item()) / P
+
+
+def main(argv=None) -> int:
+    """Flat-start reconstruction of a named synthetic workload; prints per-epoch loss / NMSE and a JSON
+    summary.  ``python -m quantem_amd.reconstruct --workload large30-1024px --epochs 50 --segment 6``."""
+    import argparse
+    import json
+    import sys
+
+    from . import synth
+
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--workload", default="single-slice", choices=sorted(synth.WORKLOADS))
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--batch", type=int, default=None, help="default: the workload's batch")
+    ap.add_argument("--segment", type=int, default=0, help="segment_slices of the backward pass")
+    ap.add_argument("--learn-probe", action="store_true", help="also optimise the probe (default: fixed)")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--out", default=None, help="write the summary JSON here")
+    args = ap.parse_args(argv)
+    if not torch.cuda.is_available():
+        print("the reconstruction runs on a GPU", file=sys.stderr)
+        return 1
+    spec = synth.WORKLOADS[args.workload]
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    prob = synth.make_problem(spec, seed=args.seed)
+    amps, mean_I = simulate_amplitudes(spec, prob, dev, args.seed)
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    print(f"[reconstruct] {spec.name}: {amps.shape[0]} patterns simulated in {t_data:.1f} s", flush=True)
+    true_phase = prob["true_phase"].sum(0)
+    crop = scan_window(spec)
+    rec = Reconstruction(prob["positions"], amps, prob["propagators"], prob["probe"],
+                         (spec.num_slices,) + tuple(spec.obj_shape), mean_I, upsample_factor=spec.scan_step,
+                         device=dev)
+    nmse_hist = []
+
+    def report(epoch, loss, r):
+        est = projected_phase(r.obj_constrained.detach()).cpu().numpy()
+        nmse_hist.append(phase_nmse(est, true_phase, crop))
+        print(f"[reconstruct] epoch {epoch + 1}: loss {loss:.5g}, phase NMSE {nmse_hist[-1]:.4f}, "
+              f"{r.times[-1]:.1f} s", flush=True)
+
+    torch.cuda.reset_peak_memory_stats(dev)
+    rec.reconstruct(args.epochs, args.batch or spec.batch, learn_probe=args.learn_probe, seed=args.seed,
+                    segment_slices=args.segment, callback=report)
+    P = amps.shape[0]
+    out = {"workload": spec.name, "patterns": P, "epochs": args.epochs, "batch": args.batch or spec.batch,
+           "segment_slices": args.segment, "learn_probe": args.learn_probe,
+           "reconstruction_s": round(rec.times[-1], 3),
+           "patterns_per_s": round(P * args.epochs / rec.times[-1], 1),
+           "final_loss": rec.history[-1], "final_phase_nmse": nmse_hist[-1],
+           "phase_nmse_per_epoch": [round(x, 5) for x in nmse_hist],
+           "peak_hbm_bytes": int(torch.cuda.max_memory_allocated(dev)), "simulation_s": round(t_data, 2)}
+    print(json.dumps(out), flush=True)
+    if args.out:
+        with open(args.out, "w") as f:
+            json.dump(out, f, indent=1)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

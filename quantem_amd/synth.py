@@ -175,10 +175,24 @@ DEPTH = ProblemSpec(
     name="multislice-20x512-256px-2modes-48x48scan",
     num_slices=20, obj_shape=(512, 512), roi=256, num_modes=2, mode_weights=(0.98, 0.02), scan=(48, 48),
     slice_thickness=1.0, n_atoms=530, batch=256)
+# large single-GPU reconstruction: 30 slices of 1024x1024 at 1.5 A, 256x256 patterns, 8 modes (the
+# reference's default weights, probe_models.py:686-689), 128x128 raster (16384 patterns), batch 512
+LARGE = ProblemSpec(
+    name="multislice-30x1024-256px-8modes-128x128scan",
+    num_slices=30, obj_shape=(1024, 1024), roi=256, num_modes=8, mode_weights=(0.86,) + (0.02,) * 7,
+    scan=(128, 128), slice_thickness=1.5, n_atoms=2133, batch=512)
 SINGLE_SLICE = ProblemSpec(
     name="single-slice-192px-64px-32x32scan",
     num_slices=1, obj_shape=(192, 192), roi=64, num_modes=1, mode_weights=(1.0,), scan=(32, 32),
     scan_step=4.0, jitter=0.5, dose=1e4, batch=64)
+
+# named workloads of bench.py and the reconstruction command line
+WORKLOADS = {
+    "headline": HEADLINE,
+    "depth20-256px": DEPTH,
+    "large30-1024px": LARGE,
+    "single-slice": SINGLE_SLICE,
+}
 
 
 def scan_origin(spec: ProblemSpec) -> tuple:
@@ -217,3 +231,4 @@ def poisson_amplitudes(intensities: np.ndarray, rng: np.random.Generator) -> tup
     counts = rng.poisson(np.clip(intensities.astype(np.float64), 0, None)).astype(np.float32)
     mean_intensity = float(counts.sum(axis=(-2, -1)).mean())
     return np.sqrt(counts).astype(np.float32), mean_intensity
+
