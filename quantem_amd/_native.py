@@ -51,6 +51,9 @@ EXPORTED_SYMBOLS = (
     "qem_conv3x3_wgrad_workspace_bytes",
     "qem_conv3x3_wgrad",
     "qem_adam_step_f32",
+    "qem_object_constraints_workspace_bytes",
+    "qem_object_constraints_forward",
+    "qem_object_constraints_backward",
 )
 
 
@@ -107,6 +110,12 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.qem_conv3x3_wgrad.restype = ctypes.c_int
     lib.qem_adam_step_f32.argtypes = [ctypes.c_int64, _vp, _vp, _vp, _vp] + [ctypes.c_double] * 6 + [i32, _vp]
     lib.qem_adam_step_f32.restype = ctypes.c_int
+    lib.qem_object_constraints_workspace_bytes.argtypes = [ctypes.c_int64, ctypes.POINTER(ctypes.c_size_t)]
+    lib.qem_object_constraints_workspace_bytes.restype = ctypes.c_int
+    lib.qem_object_constraints_forward.argtypes = [ctypes.c_int64, i32, _vp, _vp, _vp, ctypes.c_size_t, _vp]
+    lib.qem_object_constraints_forward.restype = ctypes.c_int
+    lib.qem_object_constraints_backward.argtypes = [ctypes.c_int64, i32, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]
+    lib.qem_object_constraints_backward.restype = ctypes.c_int
     lib.qem_abi_version.argtypes = []
     lib.qem_abi_version.restype = ctypes.c_int32
 

@@ -20,6 +20,7 @@ from typing import Callable, Optional
 import numpy as np
 import torch
 
+from .constraints import object_constraints
 from .optim import Adam
 from .ptychography import analytical_gradients, multislice_loss
 
@@ -51,7 +52,8 @@ class SimpleBatcher:
 
 
 def object_hard_constraints(obj: torch.Tensor, obj_type: str = "complex") -> torch.Tensor:
-    """ObjectConstraints.apply_hard_constraints for the default constraint set (object_models.py:289-339)."""
+    """ObjectConstraints.apply_hard_constraints for the default constraint set (object_models.py:289-339),
+    in torch (the loop uses :func:`quantem_amd.constraints.object_constraints` on the GPU)."""
     if obj_type in ("complex", "pure_phase"):
         amp = torch.clamp(torch.abs(obj), 0.0, 1.0) if obj_type == "complex" else 1.0
         ang = obj.angle()
@@ -137,6 +139,8 @@ class Reconstruction:
 
     @property
     def obj_constrained(self) -> torch.Tensor:
+        if self.device.type == "cuda" and self.obj_type in ("complex", "pure_phase"):
+            return object_constraints(self.obj, self.obj_type)  # the HIP kernel pair
         return object_hard_constraints(self.obj, self.obj_type)
 
     @property
