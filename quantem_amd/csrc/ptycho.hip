@@ -1025,6 +1025,26 @@ int launch_projection_update(const qem::Params& p, const Layout& L, float* overl
   return check_launch("probe_grad");
 }
 
+int launch_projection_update256(const qem::Params& p, const Layout& L, float* overlap, float* energy,
+                                hipStream_t st) {
+  hipLaunchKernelGGL(qem::k_projection<256>, dim3(L.nloss), dim3(256), 0, st, p);
+  if (int rc = check_launch("projection")) return rc;
+  if (overlap != nullptr) {
+    dim3 g((p.W + 31) / 32, (p.H + 31) / 32, p.S * p.M);
+    hipLaunchKernelGGL(qem::k_overlap<256>, g, dim3(256), 0, st, p, overlap);
+    if (int rc = check_launch("overlap")) return rc;
+  }
+  if (energy != nullptr) {
+    hipLaunchKernelGGL(qem::k_patch_energy<256>, dim3(p.S * p.B), dim3(256), 0, st, p, energy);
+    if (int rc = check_launch("patch_energy")) return rc;
+  }
+  if (int rc = qem::launch_chain256_adjoint(p, st)) return rc;
+  if (int rc = launch_obj_reduce<256>(p, st)) return rc;
+  hipLaunchKernelGGL(qem::k_probe_reduce<256>, dim3(p.M * 256 * 256 / 64), dim3(256), 0, st, p);
+  if (int rc = check_launch("probe_reduce")) return rc;
+  return qem::launch_probe_grad256(p, st);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1194,7 +1214,6 @@ int qem_ptycho_projection_update(const qem_ptycho_desc* desc, const void* obj, c
   qem_ptycho_desc d = *desc;
   if (!(d.flags & QEM_FLAG_STORE_WAVES))
     return fail(QEM_ERR_ARG, "projection update needs a forward pass run with QEM_FLAG_STORE_WAVES");
-  if (d.roi == 256) return fail(QEM_ERR_UNSUPPORTED, "projection update is implemented for roi <= 128");
   if (layout_of(d).ck != 0)
     return fail(QEM_ERR_UNSUPPORTED, "projection update needs every incident wave stored (segment_slices 0)");
   if (int rc = check_ptr(obj, "obj", true)) return rc;
@@ -1227,7 +1246,8 @@ int qem_ptycho_projection_update(const qem_ptycho_desc* desc, const void* obj, c
   switch (d.roi) {
     case 32: return launch_projection_update<32>(p, L, probe_overlap, patch_energy, st);
     case 64: return launch_projection_update<64>(p, L, probe_overlap, patch_energy, st);
-    default: return launch_projection_update<128>(p, L, probe_overlap, patch_energy, st);
+    case 128: return launch_projection_update<128>(p, L, probe_overlap, patch_energy, st);
+    default: return launch_projection_update256(p, L, probe_overlap, patch_energy, st);
   }
 }
 

@@ -271,7 +271,16 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(1, 1))) void 
   const int q = obj_col_base(c0, p.W);
   const bool top = p.s1 == p.S;
 
-  if (top) {
+  const bool analytic = (p.flags & QEM_FLAG_ANALYTIC) != 0;
+  if (top && analytic) {
+    // the projection kernel left the Fourier-space residual X' - X over X: g = ifft2_ortho(residual)
+    const float inv_n = 1.0f / (float)N;
+    col_pass(Xs, Wb, sm, [&](int, cf(&v)[16]) {
+#pragma unroll
+      for (int e = 0; e < 16; ++e) v[e] = cscale(v[e], inv_n);
+      fft256<true, true>(v, sm);
+    });
+  } else if (top) {
     // g_x = (1/N) F^H (2 dL/dI X): inverse column DFT here, inverse row DFT in the first row pass
     const float* gI = p.gI + (size_t)b * N2;
     const float sc = 2.0f * p.dloss[0] / (float)N;
@@ -359,6 +368,11 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(1, 1))) void 
     }
   }
   if (p.s0 > 0) return;
+  if (analytic) {
+    // the reference sums g at the probe plane without the shift's adjoint: G = fft2(g_psi_0) over X
+    col_pass(Wb, Xs, sm, [&](int, cf(&v)[16]) { fft256<false, true>(v, sm); });
+    return;
+  }
   // G = fft2(g_psi_0); probe-gradient spectrum conj(ramp_b) G over X, position partials
   // (2 pi / N^2) sum f Im(conj(G) P_m ramp_b)
   const cf* pk = p.pk + (size_t)m * N2;

@@ -288,8 +288,6 @@ def analytical_gradients(
     device = obj.device
     S, H, W = obj.shape
     M, N, _ = probe.shape
-    if N == 256:
-        raise ValueError("the analytical update is implemented for roi <= 128")
     B = positions.shape[0]
     obj_c = _need(obj.detach(), "obj", torch.complex64, (S, H, W), device)
     probe_c = _need(probe.detach(), "probe", torch.complex64, (M, N, N), device)

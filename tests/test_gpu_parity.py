@@ -224,7 +224,7 @@ def test_argument_errors(cuda):
         qp.multislice_loss(obj, probe, pos, props, tg, num_gpts=8, mean_intensity=0.0)
 
 
-@pytest.mark.parametrize("n,s,m,b", [(32, 1, 1, 5), (64, 3, 2, 4), (128, 4, 3, 3)])
+@pytest.mark.parametrize("n,s,m,b", [(32, 1, 1, 5), (64, 3, 2, 4), (128, 4, 3, 3), (256, 3, 2, 2), (256, 1, 1, 2)])
 def test_analytical_update_matches_oracle(cuda, n, s, m, b):
     """The reference's autograd=False update (Fourier projection + slice replay)."""
     print(f"analytical n={n} s={s} m={m}", flush=True)
