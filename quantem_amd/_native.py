@@ -54,6 +54,9 @@ EXPORTED_SYMBOLS = (
     "qem_object_constraints_workspace_bytes",
     "qem_object_constraints_forward",
     "qem_object_constraints_backward",
+    "qem_pattern_com",
+    "qem_center_amplitudes_workspace_bytes",
+    "qem_center_amplitudes",
 )
 
 
@@ -116,6 +119,12 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.qem_object_constraints_forward.restype = ctypes.c_int
     lib.qem_object_constraints_backward.argtypes = [ctypes.c_int64, i32, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]
     lib.qem_object_constraints_backward.restype = ctypes.c_int
+    lib.qem_pattern_com.argtypes = [i32, i32, _vp, _vp, _vp, _vp]
+    lib.qem_pattern_com.restype = ctypes.c_int
+    lib.qem_center_amplitudes_workspace_bytes.argtypes = [i32, i32, ctypes.POINTER(ctypes.c_size_t)]
+    lib.qem_center_amplitudes_workspace_bytes.restype = ctypes.c_int
+    lib.qem_center_amplitudes.argtypes = [i32, i32, _vp, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]
+    lib.qem_center_amplitudes.restype = ctypes.c_int
     lib.qem_abi_version.argtypes = []
     lib.qem_abi_version.restype = ctypes.c_int32
 

@@ -417,6 +417,54 @@ __global__ __launch_bounds__(T) void k_probe_grad256(Params p) {
   });
 }
 
+// Centring of measured amplitudes (preprocess.hip, n = 256): for each pattern, a = sqrt(max(I, 0))
+// into W, forward row DFT, column DFT x shift ramp x inverse column DFT, inverse row DFT and the real
+// part / N^2, clamped at 0, written fftshifted.  Workgroups loop over the patterns with their own W.
+__global__ __launch_bounds__(T) void k_center256(int P, const float* __restrict__ I, const float* __restrict__ com,
+                                                 float* __restrict__ out, float* __restrict__ sums, cf* ws) {
+  __shared__ Smem sm;
+  init_tw(sm);
+  cf* Wb = ws + (size_t)blockIdx.x * N2;
+  const float inv_n2 = 1.0f / (float)N2;
+  for (int pat = blockIdx.x; pat < P; pat += gridDim.x) {
+    const float* src = I + (size_t)pat * N2;
+    double si = 0.0, sa = 0.0;
+    for (int i = threadIdx.x; i < N2; i += T) {
+      const float x = fmaxf(src[i], 0.0f);
+      const float a = sqrtf(x);
+      si += (double)x;
+      sa += (double)a;
+      Wb[i] = mk(a, 0.0f);
+    }
+    __syncthreads();  // W complete before the row pass reads it
+    row_pass<false, true>(Wb, Wb, sm, NoOp());
+    const float dr = -com[2 * pat], dc = -com[2 * pat + 1];
+    col_pass(Wb, Wb, sm, [&](int kc, cf(&v)[16]) {
+      fft256<false, true>(v, sm);
+      const cf pc = expm2pi(fftfreq1<N>(kc) * dc);
+      const int jj = col_j();
+#pragma unroll
+      for (int e = 0; e < 16; ++e) v[e] = cmul(v[e], cmul(expm2pi(fftfreq1<N>(jj + 16 * e) * dr), pc));
+      fft256<true, true>(v, sm);
+    });
+    float* dst = out + (size_t)pat * N2;
+    row_pass<true, false>(Wb, nullptr, sm, [&](int r, cf(&v)[16]) {
+      const int rs = (r + N / 2) & (N - 1);
+      const int j = row_j();
+#pragma unroll
+      for (int e = 0; e < 16; ++e) dst[rs * N + ((j + 16 * e + N / 2) & (N - 1))] = fmaxf(v[e].x * inv_n2, 0.0f);
+    });
+    if (sums != nullptr) {
+      const double ti = block_sum<T>(si, sm.red);
+      const double ta = block_sum<T>(sa, sm.red);
+      if (threadIdx.x == 0) {
+        sums[2 * pat] = (float)ti;
+        sums[2 * pat + 1] = (float)ta;
+      }
+    }
+  }
+}
+
 }  // namespace n256
 
 int launch_chain256_forward(const Params& p, hipStream_t st) {
@@ -444,6 +492,12 @@ int launch_chain256_adjoint(const Params& p, hipStream_t st) {
 int launch_probe_grad256(const Params& p, hipStream_t st) {
   hipLaunchKernelGGL(n256::k_probe_grad256, dim3(p.M), dim3(n256::T), 0, st, p);
   return check_last_launch("probe_grad256");
+}
+
+int launch_center256(int P, int blocks, const float* I, const float* com, float* out, float* sums, cf* ws,
+                     hipStream_t st) {
+  hipLaunchKernelGGL(n256::k_center256, dim3(blocks), dim3(n256::T), 0, st, P, I, com, out, sums, ws);
+  return check_last_launch("center256");
 }
 
 }  // namespace qem

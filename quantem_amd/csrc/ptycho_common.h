@@ -130,5 +130,9 @@ int launch_chain256_forward(const Params& p, hipStream_t st);   // probe spectra
 int launch_forward256_kernel(const Params& p, hipStream_t st);  // forward kernel only (p.fwd_mode)
 int launch_chain256_adjoint(const Params& p, hipStream_t st);
 int launch_probe_grad256(const Params& p, hipStream_t st);
+// Amplitude centring of measured 256 x 256 patterns (preprocess.hip), `blocks` workgroups with one
+// 256 x 256 complex64 wave each in ws.
+int launch_center256(int P, int blocks, const float* I, const float* com, float* out, float* sums, cf* ws,
+                     hipStream_t st);
 
 }  // namespace qem
