@@ -91,3 +91,55 @@ def preprocess(intensities: torch.Tensor, scan_shape, mask: torch.Tensor | None 
     s = sums.double().sum(dim=0).cpu().numpy() / intensities.shape[0]
     return {"targets": centered, "mean_intensity": float(s[0]), "mean_amplitude": float(s[1]),
             "com_measured": com, "com_fit": com_fit, "descan_shifts": (n / 2.0 - com_fit).astype(np.float32)}
+
+
+def _rotate(com_r: np.ndarray, com_c: np.ndarray, angle: np.ndarray, transpose: bool) -> tuple:
+    """Rotate CoM vectors by `angle` (radians, broadcastable), the transpose swapping the components first."""
+    a_r, a_c = (com_c, com_r) if transpose else (com_r, com_c)
+    ca, sa = np.cos(angle), np.sin(angle)
+    return ca * a_r - sa * a_c, sa * a_r + ca * a_c
+
+
+def _mean_abs_curl(rot_r: np.ndarray, rot_c: np.ndarray) -> np.ndarray:
+    """Mean |d rot_c / d r - d rot_r / d c| over the interior of the scan (central differences)."""
+    d_rc = rot_r[..., 1:-1, 2:] - rot_r[..., 1:-1, :-2]
+    d_cr = rot_c[..., 2:, 1:-1] - rot_c[..., :-2, 1:-1]
+    return np.mean(np.abs(d_cr - d_rc), axis=(-2, -1))
+
+
+def com_rotation(com_measured: np.ndarray, com_fit: np.ndarray, scan_shape, reciprocal_sampling=(1.0, 1.0),
+                 angles_deg: np.ndarray | None = None, force_rotation_deg: float | None = None,
+                 force_transpose: bool | None = None) -> dict:
+    """Relative rotation (and transpose) between the scan and the detector that makes the CoM field
+    curl-free, as ``_set_com_relative_rotation`` (dataset_models.py:1113-1371, without the plots).
+
+    com_measured, com_fit: (P, 2) over a row-major (Rr, Rc) scan.  The normalised CoM is
+    nan_to_num(measured - fit) * reciprocal_sampling (dataset_models.py:824-828); the search minimises
+    the mean absolute curl over `angles_deg` (default -89 .. 89 degrees in steps of 1) for both
+    transpose options unless forced.  Returns the rotation (radians), the transpose flag, the curl
+    curves and the rotated CoM (2, Rr, Rc)."""
+    Rr, Rc = scan_shape
+    d = np.nan_to_num(np.asarray(com_measured, dtype=np.float64) - np.asarray(com_fit, dtype=np.float64))
+    com_r = d[:, 0].reshape(Rr, Rc) * reciprocal_sampling[0]
+    com_c = d[:, 1].reshape(Rr, Rc) * reciprocal_sampling[1]
+    angles_deg = np.arange(-89.0, 90.0, 1.0) if angles_deg is None else np.asarray(angles_deg, dtype=np.float64)
+    curls = {}
+    if force_rotation_deg is not None:
+        best = np.deg2rad(force_rotation_deg)
+        if force_transpose is None:
+            c0 = float(_mean_abs_curl(*_rotate(com_r, com_c, best, False)))
+            c1 = float(_mean_abs_curl(*_rotate(com_r, com_c, best, True)))
+            transpose = c1 < c0
+        else:
+            transpose = bool(force_transpose)
+    else:
+        ang = np.deg2rad(angles_deg)[:, None, None]
+        options = [bool(force_transpose)] if force_transpose is not None else [False, True]
+        for t in options:
+            curls[t] = _mean_abs_curl(*_rotate(com_r[None], com_c[None], ang, t))
+        # smallest curl; the untransposed option wins ties
+        t_best = min(options, key=lambda t: (float(curls[t].min()), t))
+        transpose = t_best
+        best = float(np.deg2rad(angles_deg[int(np.argmin(curls[t_best]))]))
+    rot = np.stack(_rotate(com_r, com_c, best, transpose))
+    return {"rotation_rad": float(best), "transpose": bool(transpose), "curl": curls, "com": rot}

@@ -99,3 +99,26 @@ def test_preprocess_end_to_end(cuda):
     assert np.allclose(out["com_fit"], fit, atol=1e-4)
     assert np.allclose(out["descan_shifts"], 32.0 - fit, atol=1e-4)
     assert out["mean_intensity"] == pytest.approx(np.maximum(I.astype(np.float64), 0).sum() / 30, rel=1e-6)
+
+
+def test_com_rotation_undoes_a_known_rotation():
+    """A curl-free CoM field (the gradient of a smooth potential) rotated by 23 degrees (and, in the
+    second case, transposed) is found to need a -23 degree rotation (and the transpose)."""
+    rr, cc = np.meshgrid(np.arange(24.0), np.arange(20.0), indexing="ij")
+    phi = np.sin(rr / 4.0) * np.cos(cc / 5.0) + 0.02 * rr * cc
+    g_r, g_c = np.gradient(phi)
+    th = np.deg2rad(23.0)
+    rot_r = np.cos(th) * g_r - np.sin(th) * g_c
+    rot_c = np.sin(th) * g_r + np.cos(th) * g_c
+    com_fit = np.zeros((24 * 20, 2))
+    for transposed in (False, True):
+        m_r, m_c = (rot_c, rot_r) if transposed else (rot_r, rot_c)  # a transpose swaps the detector axes
+        meas = np.stack([m_r.ravel(), m_c.ravel()], axis=1)
+        out = qpre.com_rotation(meas, com_fit, (24, 20))
+        assert out["transpose"] == transposed
+        # with the transpose the swap restores (rot_r, rot_c), so -23 degrees undoes the rotation in both cases
+        assert np.rad2deg(out["rotation_rad"]) == pytest.approx(-23.0, abs=0.51)
+        curl_best = min(float(v.min()) for v in out["curl"].values())
+        assert curl_best < 1e-2 * max(float(v.max()) for v in out["curl"].values())
+    forced = qpre.com_rotation(np.stack([rot_r.ravel(), rot_c.ravel()], 1), com_fit, (24, 20), force_rotation_deg=-23.0)
+    assert forced["transpose"] is False
