@@ -211,7 +211,7 @@ def multislice_loss(
         loss (0-d float32 tensor), and intensities if requested.
     """
     if loss_type not in nat.LOSS_TYPES:
-        raise ValueError(f"Unknown loss type {loss_type}, should be 'l1' or 'l2'")
+        raise ValueError(f"Unknown loss type {loss_type}, should be one of {sorted(nat.LOSS_TYPES)}")
     if int(segment_slices) < 0:
         raise ValueError(f"segment_slices must be >= 0, got {segment_slices}")
     loss, intens = MultisliceLoss.apply(obj, probe, positions, propagators, targets, mask, descan, num_gpts,

@@ -9,7 +9,7 @@ scanned field of view.  Every 3x3 convolution (forward, input gradient, weight g
 matrix cores through ``libqem_ptycho.so`` (bf16 operands, fp32 accumulation); pooling, upsampling,
 ReLU and concatenation are torch plumbing.  The network is trained only on pairs simulated by this
 package's multislice forward kernel, and the plain flat-object reconstruction remains the fallback
-and the judge of the warm start's error (:func:`evaluate`).
+and the yardstick of the warm start's error (:func:`evaluate`).
 """
 
 from __future__ import annotations
