@@ -29,26 +29,84 @@ def _real_view(x: torch.Tensor) -> torch.Tensor:
     return torch.view_as_real(x) if x.is_complex() else x
 
 
-class SimpleBatcher:
-    """Random mini-batches of scan indices, a new permutation every epoch (ptycho_utils.py:16-117)."""
+def validation_split(indices: np.ndarray, val_ratio: float, val_mode: str, rng: np.random.Generator):
+    """(train, validation) index arrays of the batcher's fixed split (``ptycho_utils.py:34-76``).
 
-    def __init__(self, num: int, batch_size: Optional[int], shuffle: bool = True, rng=None):
+    ``val_ratio`` outside [0, 1) means no validation set; ``round(num * val_ratio)`` indices are held
+    out.  "random" takes them from one permutation drawn from ``rng``.  "grid" takes every k-th index,
+    k = round(1 / val_ratio), as validation for ratios up to 1/2; above 1/2 every k-th index,
+    k = round(1 / (1 - val_ratio)), is the training set and the rest validation.  Either grid selection
+    keeps at most ``round(num * val_ratio)`` of the strided indices.
+    """
+    empty = np.asarray([], dtype=int)
+    if not 0.0 <= val_ratio < 1.0:
+        val_ratio = 0.0
+    n_val = int(round(len(indices) * val_ratio))
+    if n_val == 0:
+        return indices, empty
+    if val_mode == "random":
+        val = rng.permutation(indices)[:n_val]
+        return np.setdiff1d(indices, val), val
+    if val_ratio <= 0.5:
+        val = indices[::max(1, int(round(1.0 / val_ratio)))][:n_val]
+        return np.setdiff1d(indices, val), val
+    train = indices[::max(1, int(round(1.0 / (1.0 - val_ratio))))][:n_val]
+    return train, np.setdiff1d(indices, train)
+
+
+class SimpleBatcher:
+    """Random mini-batches of scan indices, a new permutation every epoch (ptycho_utils.py:16-117).
+
+    An optional validation set, fixed for the batcher's lifetime, is either given explicitly
+    (``train_indices`` with ``val_indices``) or split off by :func:`validation_split`; iteration
+    covers the training indices only and :meth:`iter_val` the validation ones, in order.
+    """
+
+    def __init__(self, num: int, batch_size: Optional[int], shuffle: bool = True, rng=None,
+                 val_ratio: float = 0.0, val_mode: str = "grid", train_indices=None, val_indices=None):
         self.indices = np.arange(num)
         self.batch_size = batch_size if batch_size is not None else num
         self.shuffle = shuffle
-        if rng is None or isinstance(rng, (int, np.integer)):
-            rng = np.random.default_rng(rng)
+        if rng is None or isinstance(rng, (int, float, np.integer)):
+            rng = np.random.default_rng(None if rng is None else int(rng))
         if not isinstance(rng, np.random.Generator):
             raise TypeError(f"rng should be a np.random.Generator or a seed, got {type(rng)}")
         self.rng = rng
+        if train_indices is not None or val_indices is not None:
+            if train_indices is None or val_indices is None:
+                raise ValueError("Both train_indices and val_indices must be provided together.")
+            self.train_indices = np.asarray(train_indices, dtype=int)
+            self.val_indices = np.asarray(val_indices, dtype=int)
+        else:
+            self.train_indices, self.val_indices = validation_split(self.indices, val_ratio, val_mode, self.rng)
 
     def __iter__(self):
-        order = self.rng.permutation(self.indices) if self.shuffle else self.indices
+        order = self.rng.permutation(self.train_indices) if self.shuffle else self.train_indices
         for i in range(0, len(order), self.batch_size):
             yield order[i:i + self.batch_size]
 
     def __len__(self) -> int:
-        return int(ceil(len(self.indices) / self.batch_size))
+        return int(ceil(len(self.train_indices) / self.batch_size))
+
+    def iter_val(self):
+        v = self.val_indices
+        return (v[i:i + self.batch_size] for i in range(0, len(v), self.batch_size))
+
+    @property
+    def has_validation(self) -> bool:
+        return len(self.val_indices) > 0
+
+    def val_len(self) -> int:
+        return int(ceil(len(self.val_indices) / self.batch_size))
+
+
+# the dataset's hard / soft constraint switches and their defaults (dataset_models.py:557-562)
+DATASET_CONSTRAINTS = {
+    "descan_tv_weight": 0.0,
+    "descan_shifts_constant": False,
+    "center_scan_positions": False,
+    "clip_scan_positions": True,
+}
 
 
 def object_hard_constraints(obj: torch.Tensor, obj_type: str = "complex") -> torch.Tensor:
@@ -101,10 +159,10 @@ class Reconstruction:
 
     def __init__(self, positions, targets, propagators, probe, obj_shape, mean_intensity: float,
                  mask=None, obj_type: str = "complex", orthogonalize: bool = True, upsample_factor: float = 1.0,
-                 obj_init: Optional[torch.Tensor] = None, device="cuda"):
+                 obj_init: Optional[torch.Tensor] = None, descan=None, device="cuda"):
         dev = torch.device(device)
         self.device = dev
-        self.positions = torch.as_tensor(positions, dtype=torch.float32, device=dev).contiguous()
+        self.positions = torch.as_tensor(positions, dtype=torch.float32, device=dev).clone().contiguous()
         self.targets = torch.as_tensor(targets, dtype=torch.float32, device=dev).contiguous()
         S = int(obj_shape[0])
         self.propagators = (torch.as_tensor(propagators, dtype=torch.complex64, device=dev).contiguous()
@@ -112,6 +170,13 @@ class Reconstruction:
         self.mask = None if mask is None else torch.as_tensor(mask, dtype=torch.float32, device=dev)
         self.mean_intensity = float(mean_intensity)
         self.num_gpts = int(self.positions.shape[0])
+        # descan shifts in pixels (P, 2); the forward model applies them only while they are learned
+        self.descan = (torch.zeros((self.num_gpts, 2), dtype=torch.float32, device=dev) if descan is None else
+                       torch.as_tensor(descan, dtype=torch.float32, device=dev).clone().contiguous())
+        if self.positions.shape != (self.num_gpts, 2) or self.descan.shape != (self.num_gpts, 2):
+            raise ValueError(f"positions and descan must be (P, 2), got {tuple(self.positions.shape)} and "
+                             f"{tuple(self.descan.shape)}")
+        self.obj_shape = tuple(int(x) for x in obj_shape)
         self.obj_type = obj_type
         self.orthogonalize = orthogonalize
         self.upsample_factor = float(upsample_factor)
@@ -127,6 +192,7 @@ class Reconstruction:
         self._probe = torch.nn.Parameter(
             _real_view(torch.as_tensor(probe, dtype=torch.complex64, device=dev)).clone())
         self.history: list = []
+        self.val_history: list = []
         self.times: list = []
 
     @property
@@ -147,18 +213,56 @@ class Reconstruction:
     def probe_constrained(self) -> torch.Tensor:
         return orthogonalize_probe(self.probe) if (self.orthogonalize and self.probe.shape[0] > 1) else self.probe
 
+    def apply_dataset_constraints(self, constraints: dict) -> None:
+        """The dataset's hard constraints on the scan positions (``dataset_models.py:597-613``), in place:
+        clip to [0, object extent - 1] per axis, then (optionally) move the mean position to the
+        object centre."""
+        ext = torch.tensor(self.obj_shape[1:], dtype=torch.float32, device=self.device)
+        with torch.no_grad():
+            pos = self.positions
+            if constraints["clip_scan_positions"]:
+                pos.copy_(torch.clamp(pos, min=torch.zeros_like(ext), max=ext - 1))
+            if constraints["center_scan_positions"]:
+                pos.copy_(pos - pos.mean(dim=0, keepdim=True) + ext / 2)
+
+    def descan_tv_loss(self, weight: float) -> torch.Tensor:
+        """weight * (mean |diff d_r| + mean |diff d_c|) / 2 over the scan order (``dataset_models.py:579-587``)."""
+        d = self.descan
+        return weight * (torch.mean(torch.abs(d[:, 0].diff())) + torch.mean(torch.abs(d[:, 1].diff()))) / 2
+
     def reconstruct(self, num_iters: int, batch_size: int, obj_lr: float = 5e-3, probe_lr: float = 1e-3,
                     learn_probe: bool = True, loss_type: str = "l2_amplitude", seed: int = 0,
-                    autograd: bool = True, segment_slices: int = 0,
+                    autograd: bool = True, segment_slices: int = 0, dataset_lr: Optional[float] = None,
+                    learn_positions: bool = True, learn_descan: bool = True,
+                    dataset_constraints: Optional[dict] = None, val_ratio: float = 0.0, val_mode: str = "grid",
                     callback: Optional[Callable[[int, float, "Reconstruction"], None]] = None) -> list:
-        """Runs `num_iters` epochs; returns the per-epoch mean batch loss.
+        """Runs `num_iters` epochs; returns the per-epoch mean batch loss (consistency + soft constraints).
 
         ``autograd=False`` selects the reference's analytical update (Fourier projection and slice
         replay, ``ptychography.py:339-372``) instead of the loss gradient; as in the reference, its
         gradients are assigned to the raw parameters without the upsample rescale.
         ``segment_slices`` k >= 2 keeps every k-th incident wave and recomputes the rest in the
         backward pass (less memory; see :func:`quantem_amd.ptychography.multislice_loss`).
+
+        ``dataset_lr`` set gives the dataset an Adam optimizer over the scan positions and / or the
+        descan shifts (``learn_positions``, ``learn_descan``; ``dataset_models.py:98-108``, default
+        rate 1e-3 in the reference).  Descan shifts enter the forward model only then
+        (``dataset_models.py:1495-1498``); ``dataset_constraints`` overrides :data:`DATASET_CONSTRAINTS`.
+        Position and descan gradients come from the loss gradient, so they need ``autograd=True``.
+        ``val_ratio`` > 0 holds out scan positions (:func:`validation_split`): they take no optimizer
+        step, and their mean batch loss after each epoch goes to ``self.val_history``.
         """
+        cons = dict(DATASET_CONSTRAINTS)
+        if dataset_constraints:
+            unknown = set(dataset_constraints) - set(cons)
+            if unknown:
+                raise KeyError(f"unknown dataset constraints {sorted(unknown)}; known: {sorted(cons)}")
+            cons.update(dataset_constraints)
+        learn_dset = dataset_lr is not None and (learn_positions or learn_descan)
+        if learn_dset and not autograd:
+            raise ValueError("learning positions or descan needs autograd=True (the analytical update gives "
+                             "no position or descan gradient)")
+        use_descan = learn_dset and learn_descan
         # the object-gradient rescale of the autograd path (ptychography.py:330-332) is applied inside
         # the optimizer step; the analytical update is not rescaled, as in the reference
         scale = self.upsample_factor ** 2 / 2 if autograd else 1.0
@@ -166,39 +270,75 @@ class Reconstruction:
         if learn_probe:
             groups.append({"params": [self._probe], "lr": probe_lr})
         self._probe.requires_grad_(learn_probe)
+        dset_params = []
+        if learn_dset and learn_descan:
+            dset_params.append(self.descan)
+        if learn_dset and learn_positions:
+            dset_params.append(self.positions)
+        for t in (self.positions, self.descan):
+            t.requires_grad_(any(t is q for q in dset_params))
+        if dset_params:
+            groups.append({"params": dset_params, "lr": float(dataset_lr)})
         opt = Adam(groups)
-        batcher = SimpleBatcher(self.num_gpts, batch_size, rng=seed)
+        batcher = SimpleBatcher(self.num_gpts, batch_size, rng=seed, val_ratio=val_ratio, val_mode=val_mode)
+        tv_weight = float(cons["descan_tv_weight"]) if use_descan else 0.0
+
+        def batch_inputs(idx):
+            it = torch.from_numpy(idx).to(self.device)
+            self.apply_dataset_constraints(cons)  # every batch, as the dataset's forward does
+            desc = None
+            if use_descan:
+                desc = torch.zeros_like(self.descan) if cons["descan_shifts_constant"] else self.descan
+                desc = desc[it]
+            return it, self.positions[it], desc
+
         t0 = time.perf_counter()
         t_cb = 0.0  # time spent in callbacks, excluded from self.times
-        for epoch in range(num_iters):
-            total = torch.zeros((), dtype=torch.float32, device=self.device)
-            for idx in batcher:
-                it = torch.from_numpy(idx).to(self.device)
-                opt.zero_grad(set_to_none=True)
-                if autograd:
-                    loss = multislice_loss(self.obj_constrained, self.probe_constrained, self.positions[it],
-                                           self.propagators, self.targets[it], self.mask, None,
-                                           num_gpts=self.num_gpts, mean_intensity=self.mean_intensity,
-                                           loss_type=loss_type, segment_slices=segment_slices)
-                    loss.backward()
-                else:
+        try:
+            for epoch in range(num_iters):
+                total = torch.zeros((), dtype=torch.float32, device=self.device)
+                for idx in batcher:
+                    opt.zero_grad(set_to_none=True)
+                    it, pos, desc = batch_inputs(idx)
+                    if autograd:
+                        loss = multislice_loss(self.obj_constrained, self.probe_constrained, pos, self.propagators,
+                                               self.targets[it], self.mask, desc, num_gpts=self.num_gpts,
+                                               mean_intensity=self.mean_intensity, loss_type=loss_type,
+                                               segment_slices=segment_slices)
+                        if tv_weight > 0:
+                            loss = loss + self.descan_tv_loss(tv_weight)
+                        loss.backward()
+                    else:
+                        with torch.no_grad():
+                            loss, og, pg = analytical_gradients(
+                                self.obj_constrained, self.probe_constrained, pos, self.propagators,
+                                self.targets[it], self.mask, None, num_gpts=self.num_gpts,
+                                mean_intensity=self.mean_intensity, loss_type=loss_type, obj_type=self.obj_type)
+                        self._obj.grad = _real_view(og).contiguous()
+                        if learn_probe:
+                            self._probe.grad = _real_view(pg).contiguous()
+                    opt.step()
+                    total = total + loss.detach()
+                if batcher.has_validation:
+                    vtot = torch.zeros((), dtype=torch.float32, device=self.device)
                     with torch.no_grad():
-                        loss, og, pg = analytical_gradients(
-                            self.obj_constrained, self.probe_constrained, self.positions[it], self.propagators,
-                            self.targets[it], self.mask, None, num_gpts=self.num_gpts,
-                            mean_intensity=self.mean_intensity, loss_type=loss_type, obj_type=self.obj_type)
-                    self._obj.grad = _real_view(og).contiguous()
-                    if learn_probe:
-                        self._probe.grad = _real_view(pg).contiguous()
-                opt.step()
-                total = total + loss.detach()
-            mean_loss = float(total.item()) / len(batcher)  # one host sync per epoch
-            self.history.append(mean_loss)
-            self.times.append(time.perf_counter() - t0 - t_cb)
-            if callback is not None:
-                tc = time.perf_counter()
-                callback(epoch, mean_loss, self)
-                t_cb += time.perf_counter() - tc
+                        for idx in batcher.iter_val():
+                            it, pos, desc = batch_inputs(idx)
+                            vtot = vtot + multislice_loss(
+                                self.obj_constrained, self.probe_constrained, pos, self.propagators,
+                                self.targets[it], self.mask, desc, num_gpts=self.num_gpts,
+                                mean_intensity=self.mean_intensity, loss_type=loss_type)
+                    self.val_history.append(float(vtot.item()) / batcher.val_len())
+                mean_loss = float(total.item()) / len(batcher)  # one host sync per epoch
+                self.history.append(mean_loss)
+                self.times.append(time.perf_counter() - t0 - t_cb)
+                if callback is not None:
+                    tc = time.perf_counter()
+                    callback(epoch, mean_loss, self)
+                    t_cb += time.perf_counter() - tc
+        finally:
+            self.positions.requires_grad_(False)
+            self.descan.requires_grad_(False)
         return self.history
 
 
@@ -249,6 +389,9 @@ def main(argv=None) -> int:
     ap.add_argument("--batch", type=int, default=None, help="default: the workload's batch")
     ap.add_argument("--segment", type=int, default=0, help="segment_slices of the backward pass")
     ap.add_argument("--learn-probe", action="store_true", help="also optimise the probe (default: fixed)")
+    ap.add_argument("--dataset-lr", type=float, default=None,
+                    help="learn the scan positions and descan shifts at this Adam rate (default: fixed)")
+    ap.add_argument("--val-ratio", type=float, default=0.0, help="fraction of positions held out for validation")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default=None, help="write the summary JSON here")
     args = ap.parse_args(argv)
@@ -278,13 +421,15 @@ def main(argv=None) -> int:
 
     torch.cuda.reset_peak_memory_stats(dev)
     rec.reconstruct(args.epochs, args.batch or spec.batch, learn_probe=args.learn_probe, seed=args.seed,
-                    segment_slices=args.segment, callback=report)
+                    segment_slices=args.segment, dataset_lr=args.dataset_lr, val_ratio=args.val_ratio,
+                    callback=report)
     P = amps.shape[0]
     out = {"workload": spec.name, "patterns": P, "epochs": args.epochs, "batch": args.batch or spec.batch,
            "segment_slices": args.segment, "learn_probe": args.learn_probe,
            "reconstruction_s": round(rec.times[-1], 3),
            "patterns_per_s": round(P * args.epochs / rec.times[-1], 1),
            "final_loss": rec.history[-1], "final_phase_nmse": nmse_hist[-1],
+           "final_val_loss": rec.val_history[-1] if rec.val_history else None,
            "phase_nmse_per_epoch": [round(x, 5) for x in nmse_hist],
            "peak_hbm_bytes": int(torch.cuda.max_memory_allocated(dev)), "simulation_s": round(t_data, 2)}
     print(json.dumps(out), flush=True)

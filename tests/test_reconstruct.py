@@ -20,6 +20,59 @@ def test_batcher_covers_every_index_each_epoch():
         rc.SimpleBatcher(5, 2, rng="x")
 
 
+def test_validation_split_grid_random_and_explicit():
+    idx = np.arange(20)
+    rng = np.random.default_rng(0)
+    # grid, ratio <= 1/2: every round(1/r)-th index is held out
+    tr, va = rc.validation_split(idx, 0.25, "grid", rng)
+    assert va.tolist() == [0, 4, 8, 12, 16]
+    assert tr.tolist() == [i for i in range(20) if i % 4]
+    tr, va = rc.validation_split(idx, 0.3, "grid", rng)  # k = 3, 6 held out of the 7 strided
+    assert va.tolist() == [0, 3, 6, 9, 12, 15]
+    # grid, ratio > 1/2: the strided indices train, the rest validate
+    tr, va = rc.validation_split(idx, 0.75, "grid", rng)
+    assert tr.tolist() == [0, 4, 8, 12, 16] and len(va) == 15
+    # random: disjoint, covering, of the requested size
+    tr, va = rc.validation_split(idx, 0.35, "random", np.random.default_rng(1))
+    assert len(va) == 7 and sorted(np.concatenate([tr, va]).tolist()) == list(range(20))
+    # no split for ratios outside [0, 1) or rounding to zero
+    for r in (0.0, 1.0, -0.5, 0.02):
+        tr, va = rc.validation_split(idx, r, "grid", rng)
+        assert tr.tolist() == list(range(20)) and len(va) == 0
+
+
+def test_batcher_validation_iteration():
+    b = rc.SimpleBatcher(20, 3, rng=0, val_ratio=0.25)
+    assert b.has_validation and b.val_len() == 2 and len(b) == 5
+    for _ in range(2):
+        assert sorted(np.concatenate(list(b)).tolist()) == b.train_indices.tolist()
+    assert [v.tolist() for v in b.iter_val()] == [[0, 4, 8], [12, 16]]
+    e = rc.SimpleBatcher(6, 4, train_indices=[0, 1, 2, 3], val_indices=[4, 5], shuffle=False)
+    assert [x.tolist() for x in e] == [[0, 1, 2, 3]] and [v.tolist() for v in e.iter_val()] == [[4, 5]]
+    with pytest.raises(ValueError):
+        rc.SimpleBatcher(6, 4, train_indices=[0, 1])
+    assert not rc.SimpleBatcher(6, 4).has_validation and rc.SimpleBatcher(6, 4).val_len() == 0
+
+
+def test_dataset_constraints_on_positions():
+    pos = np.array([[-3.0, 5.0], [10.0, 200.0], [40.0, 50.0]], dtype=np.float32)
+    rec = rc.Reconstruction(pos, np.zeros((3, 32, 32), np.float32), None, np.ones((1, 32, 32), np.complex64),
+                            (1, 64, 128), 1.0, device="cpu")
+    rec.apply_dataset_constraints(dict(rc.DATASET_CONSTRAINTS))
+    assert rec.positions.tolist() == [[0.0, 5.0], [10.0, 127.0], [40.0, 50.0]]
+    rec.apply_dataset_constraints(dict(rc.DATASET_CONSTRAINTS, clip_scan_positions=False,
+                                       center_scan_positions=True))
+    assert np.allclose(rec.positions.mean(0).numpy(), [32.0, 64.0])
+    d = torch.tensor([[0.0, 1.0], [2.0, 1.0], [2.0, 4.0]])
+    rec.descan = d
+    # mean |diff| per axis: rows (2 + 0) / 2 = 1, columns (0 + 3) / 2 = 1.5
+    assert float(rec.descan_tv_loss(0.5)) == pytest.approx(0.5 * (1.0 + 1.5) / 2)
+    with pytest.raises(KeyError):
+        rec.reconstruct(1, 3, dataset_constraints={"clip_positions": True})
+    with pytest.raises(ValueError):
+        rec.reconstruct(1, 3, dataset_lr=1e-3, autograd=False)
+
+
 def test_object_constraints():
     obj = torch.ones((2, 5, 5), dtype=torch.complex64)
     assert torch.allclose(rc.object_hard_constraints(obj), obj)
@@ -130,3 +183,62 @@ def test_single_slice_reference_case(cuda):
     # error is the informative check (measured 0.238)
     assert hist[-1] < 0.95 * hist[0]
     assert nmse < 0.4
+
+
+def _small_problem(cuda, descan_true=None):
+    spec = synth.ProblemSpec(name="single-slice-test", num_slices=1, obj_shape=(128, 128), roi=64, num_modes=1,
+                             mode_weights=(1.0,), scan=(12, 12), scan_step=4.0, jitter=0.0, dose=1e6, n_atoms=60,
+                             batch=32)
+    prob = synth.make_problem(spec, seed=1)
+    from quantem_amd.ptychography import forward_intensities
+
+    obj = torch.from_numpy(prob["obj"]).to(cuda)
+    probe = torch.from_numpy(prob["probe"]).to(cuda)
+    pos = torch.from_numpy(prob["positions"]).to(cuda)
+    d = None if descan_true is None else torch.as_tensor(descan_true, dtype=torch.float32, device=cuda)
+    I = forward_intensities(obj, probe, pos, None, d).cpu().numpy()
+    amps, mean_I = synth.poisson_amplitudes(I, np.random.default_rng(2))
+    return spec, prob, amps, mean_I
+
+
+@pytest.mark.gpu
+def test_position_refinement_recovers_perturbed_positions(cuda):
+    """Known object and probe, scan positions perturbed by up to +-1.5 px: the dataset optimizer moves
+    them back towards the true ones (position gradient of the kernel through the loop)."""
+    spec, prob, amps, mean_I = _small_problem(cuda)
+    true = prob["positions"]
+    rng = np.random.default_rng(5)
+    start = (true + rng.uniform(-1.5, 1.5, size=true.shape)).astype(np.float32)
+    rec = rc.Reconstruction(start, amps, None, prob["probe"], (1,) + spec.obj_shape, mean_I,
+                            upsample_factor=spec.scan_step, obj_init=prob["obj"], device=cuda)
+    err0 = float(np.sqrt(np.mean((start - true) ** 2)))
+    hist = rec.reconstruct(20, 32, obj_lr=0.0, learn_probe=False, seed=0, dataset_lr=2e-2, learn_descan=False)
+    est = rec.positions.detach().cpu().numpy()
+    err = float(np.sqrt(np.mean((est - true) ** 2)))
+    print(f"position refinement: rms error {err0:.3f} -> {err:.3f} px, loss {hist[0]:.4g} -> {hist[-1]:.4g}",
+          flush=True)
+    assert not rec.positions.requires_grad
+    assert hist[-1] < hist[0]
+    assert err < 0.5 * err0
+
+
+@pytest.mark.gpu
+def test_descan_refinement_and_validation_loss(cuda):
+    """Data simulated with a known descan ramp; learning the descan shifts from zero reduces the descan
+    error and the training loss, and the held-out positions' loss is recorded every epoch."""
+    n = 144
+    ramp = np.stack([np.linspace(-1.0, 1.0, n), np.linspace(0.8, -0.8, n)], axis=1).astype(np.float32)
+    spec, prob, amps, mean_I = _small_problem(cuda, ramp)
+    rec = rc.Reconstruction(prob["positions"], amps, None, prob["probe"], (1,) + spec.obj_shape, mean_I,
+                            upsample_factor=spec.scan_step, obj_init=prob["obj"], device=cuda)
+    hist = rec.reconstruct(15, 32, obj_lr=0.0, learn_probe=False, seed=0, dataset_lr=2e-2, learn_positions=False,
+                           val_ratio=0.25, dataset_constraints={"descan_tv_weight": 1e-3})
+    d = rec.descan.detach().cpu().numpy()
+    err0 = float(np.sqrt(np.mean(ramp ** 2)))
+    err = float(np.sqrt(np.mean((d - ramp) ** 2)))
+    print(f"descan refinement: rms error {err0:.3f} -> {err:.3f} px, loss {hist[0]:.4g} -> {hist[-1]:.4g}, "
+          f"validation {rec.val_history[0]:.4g} -> {rec.val_history[-1]:.4g}", flush=True)
+    assert len(rec.val_history) == 15 and np.isfinite(rec.val_history).all()
+    assert np.allclose(rec.positions.cpu().numpy(), prob["positions"])  # positions not learned
+    assert hist[-1] < hist[0]
+    assert err < 0.5 * err0
