@@ -142,41 +142,49 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
 
 // ---------------------------------------------------------------------------------------------
 // Loss: I = sum_m |X_m|^2 (unshifted k), compared with fftshifted targets; writes dL/dI and
-// per-block partial sums.  256 threads per block, one frequency pixel per thread.
+// per-block partial sums.  256 threads per block, one frequency pixel per thread in each of the
+// block's kLossTiles consecutive 256-pixel tiles (fewer partials for the single-block finalize).
+constexpr int kLossTiles = 8;
+
 template <int N>
-__global__ __launch_bounds__(256) void k_loss(Params p) {
+__global__ __launch_bounds__(256) void k_loss(Params p, int ntiles) {
   __shared__ double red[4];
-  const int idx = blockIdx.x * 256 + threadIdx.x;  // over B*N*N (multiple of 256)
-  const int b = idx / (N * N);
-  const int k = idx - b * N * N;
-  const int kr = k / N, kc = k - kr * N;
-  float I = 0.0f;
-  for (int m = 0; m < p.M; ++m) I += cabs2(p.X[((size_t)b * p.M + m) * N * N + k]);
-  const int ks = ((kr + N / 2) & (N - 1)) * N + ((kc + N / 2) & (N - 1));
-  if (p.intens_out != nullptr) p.intens_out[(size_t)b * N * N + ks] = I;
-  const float t = p.targets[(size_t)b * N * N + ks];
   const bool amp = (p.loss_type == QEM_LOSS_L2_AMPLITUDE || p.loss_type == QEM_LOSS_L1_AMPLITUDE);
-  const float pred = amp ? sqrtf(I + 1e-9f) : I;
-  float term, dpred;
-  if (p.loss_type == QEM_LOSS_POISSON) {
-    term = pred - t * logf(pred + 1e-6f);
-    dpred = 1.0f - t / (pred + 1e-6f);
-  } else {
-    const float mk_ = p.mask[ks];
-    const float d = pred * mk_ - t * mk_;
-    if (p.loss_type == QEM_LOSS_L2_AMPLITUDE || p.loss_type == QEM_LOSS_L2_INTENSITY) {
-      term = d * d;
-      dpred = 2.0f * d * mk_;
-    } else {
-      term = fabsf(d);
-      dpred = (d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f)) * mk_;
-    }
-  }
   const float scale = (p.loss_type == QEM_LOSS_POISSON) ? 1.0f / p.mean_I
                                                           : (float)p.P / ((float)p.B * p.mean_I);
-  const float dI = amp ? dpred * 0.5f / pred : dpred;
-  p.gI[(size_t)b * N * N + k] = scale * dI;
-  const double tot = block_sum<256>((double)term, red);
+  double acc = 0.0;
+  // each thread sums its terms in tile order, then the block sums the threads
+  for (int tile = blockIdx.x * kLossTiles; tile < ntiles && tile < (blockIdx.x + 1) * kLossTiles; ++tile) {
+    const int idx = tile * 256 + threadIdx.x;
+    const int b = idx / (N * N);
+    const int k = idx - b * N * N;
+    const int kr = k / N, kc = k - kr * N;
+    float I = 0.0f;
+    for (int m = 0; m < p.M; ++m) I += cabs2(p.X[((size_t)b * p.M + m) * N * N + k]);
+    const int ks = ((kr + N / 2) & (N - 1)) * N + ((kc + N / 2) & (N - 1));
+    if (p.intens_out != nullptr) p.intens_out[(size_t)b * N * N + ks] = I;
+    const float t = p.targets[(size_t)b * N * N + ks];
+    const float pred = amp ? sqrtf(I + 1e-9f) : I;
+    float term, dpred;
+    if (p.loss_type == QEM_LOSS_POISSON) {
+      term = pred - t * logf(pred + 1e-6f);
+      dpred = 1.0f - t / (pred + 1e-6f);
+    } else {
+      const float mk_ = p.mask[ks];
+      const float d = pred * mk_ - t * mk_;
+      if (p.loss_type == QEM_LOSS_L2_AMPLITUDE || p.loss_type == QEM_LOSS_L2_INTENSITY) {
+        term = d * d;
+        dpred = 2.0f * d * mk_;
+      } else {
+        term = fabsf(d);
+        dpred = (d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f)) * mk_;
+      }
+    }
+    const float dI = amp ? dpred * 0.5f / pred : dpred;
+    p.gI[(size_t)b * N * N + k] = scale * dI;
+    acc += (double)term;
+  }
+  const double tot = block_sum<256>(acc, red);
   if (threadIdx.x == 0) p.loss_part[blockIdx.x] = tot;
 }
 
@@ -884,9 +892,10 @@ int launch_forward(const qem::Params& p, const Layout& L, hipStream_t st) {
     hipLaunchKernelGGL((qem::k_forward<N, false>), dim3(p.B * p.M), dim3(F::T), 0, st, p);
   prof_mark(false, 1, st);
   if (int rc = check_launch("forward")) return rc;
-  hipLaunchKernelGGL(qem::k_loss<N>, dim3(L.nloss), dim3(256), 0, st, p);
+  const int nparts = (L.nloss + qem::kLossTiles - 1) / qem::kLossTiles;
+  hipLaunchKernelGGL(qem::k_loss<N>, dim3(nparts), dim3(256), 0, st, p, L.nloss);
   if (int rc = check_launch("loss")) return rc;
-  hipLaunchKernelGGL(qem::k_loss_finalize, dim3(1), dim3(256), 0, st, p, L.nloss);
+  hipLaunchKernelGGL(qem::k_loss_finalize, dim3(1), dim3(256), 0, st, p, nparts);
   return check_launch("loss_finalize");
 }
 
@@ -894,9 +903,10 @@ int launch_forward256(const qem::Params& p, const Layout& L, hipStream_t st) {
   prof_mark(false, 0, st);
   if (int rc = qem::launch_chain256_forward(p, st)) return rc;
   prof_mark(false, 1, st);
-  hipLaunchKernelGGL(qem::k_loss<256>, dim3(L.nloss), dim3(256), 0, st, p);
+  const int nparts = (L.nloss + qem::kLossTiles - 1) / qem::kLossTiles;
+  hipLaunchKernelGGL(qem::k_loss<256>, dim3(nparts), dim3(256), 0, st, p, L.nloss);
   if (int rc = check_launch("loss")) return rc;
-  hipLaunchKernelGGL(qem::k_loss_finalize, dim3(1), dim3(256), 0, st, p, L.nloss);
+  hipLaunchKernelGGL(qem::k_loss_finalize, dim3(1), dim3(256), 0, st, p, nparts);
   return check_launch("loss_finalize");
 }
 
