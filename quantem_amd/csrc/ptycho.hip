@@ -143,10 +143,11 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
 // ---------------------------------------------------------------------------------------------
 // Loss: I = sum_m |X_m|^2 (unshifted k), compared with fftshifted targets; writes dL/dI and
 // per-block partial sums.  256 threads per block, one frequency pixel per thread in each of the
-// block's kLossTiles consecutive 256-pixel tiles (fewer partials for the single-block finalize).
+// block's kLossTiles consecutive 256-pixel tiles (fewer partials for the single-block finalize);
+// MC = min(M, 4) probe modes loaded together.
 constexpr int kLossTiles = 8;
 
-template <int N>
+template <int N, int MC>
 __global__ __launch_bounds__(256) void k_loss(Params p, int ntiles) {
   __shared__ double red[4];
   const bool amp = (p.loss_type == QEM_LOSS_L2_AMPLITUDE || p.loss_type == QEM_LOSS_L1_AMPLITUDE);
@@ -160,7 +161,14 @@ __global__ __launch_bounds__(256) void k_loss(Params p, int ntiles) {
     const int k = idx - b * N * N;
     const int kr = k / N, kc = k - kr * N;
     float I = 0.0f;
-    for (int m = 0; m < p.M; ++m) I += cabs2(p.X[((size_t)b * p.M + m) * N * N + k]);
+    for (int m0 = 0; m0 < p.M; m0 += MC) {  // MC loads in flight; mode indices past M - 1 are not added
+      cf x[MC];
+#pragma unroll
+      for (int j = 0; j < MC; ++j) x[j] = p.X[((size_t)b * p.M + (m0 + j < p.M ? m0 + j : p.M - 1)) * N * N + k];
+#pragma unroll
+      for (int j = 0; j < MC; ++j)
+        if (m0 + j < p.M) I += cabs2(x[j]);
+    }
     const int ks = ((kr + N / 2) & (N - 1)) * N + ((kc + N / 2) & (N - 1));
     if (p.intens_out != nullptr) p.intens_out[(size_t)b * N * N + ks] = I;
     const float t = p.targets[(size_t)b * N * N + ks];
@@ -556,7 +564,8 @@ __global__ __launch_bounds__(256) void k_obj_reduce(Params p) {
 // contiguous bytes per mode), sums the modes in order and adds the sum into a W-wide LDS row.  The
 // columns of one patch are distinct (W >= N), those of consecutive patches may coincide, so a barrier
 // separates the adds of consecutive patterns: every object pixel sums its patterns in increasing b.
-template <int N>
+// MC: probe modes whose loads are issued together (min(M, 4)).
+template <int N, int MC>
 __global__ __launch_bounds__(N) void k_obj_reduce_rows(Params p) {
   extern __shared__ cf acc_row[];  // p.W entries
   __shared__ int lb[N], lrow[N], lc0[N];
@@ -597,25 +606,39 @@ __global__ __launch_bounds__(N) void k_obj_reduce_rows(Params p) {
     for (int i0 = 0; i0 < cnt; i0 += K) {
       cf val[K];
       int xo[K];
+      size_t base[K];  // patch row of pattern i0 + k (of pattern i0 past the list's end: loaded, unused)
 #pragma unroll
       for (int k = 0; k < K; ++k) {
+        const int i = i0 + k < cnt ? i0 + k : i0;
+        base[k] = ((size_t)lb[i] * p.M * p.wslots + slot) * N * N + (size_t)lrow[i] * N + t;
+        int x = lc0[i] + x_off;  // in [-N/2, W + N/2): one wrap each way
+        x += x < 0 ? p.W : 0;
+        x -= x >= p.W ? p.W : 0;
+        xo[k] = x;
         val[k] = mk(0.0f, 0.0f);
-        xo[k] = 0;
-        if (i0 + k < cnt) {
-          const int bb = lb[i0 + k];
-          const cf* g = p.psi + ((size_t)bb * p.M * p.wslots + slot) * N * N + (size_t)lrow[i0 + k] * N + t;
-          cf acc = ld_stream(g);
-          for (int m = 1; m < p.M; ++m) {
-            const cf v = ld_stream(g + (size_t)m * slab);
-            acc.x += v.x;
-            acc.y += v.y;
+      }
+      // the modes MC at a time: every load of a group is issued before its first add (mode indices
+      // past M - 1 re-read mode M - 1 and are not added), and the adds run in mode order
+      for (int m0 = 0; m0 < p.M; m0 += MC) {
+        cf v[K][MC];
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+#pragma unroll
+          for (int j = 0; j < MC; ++j) {
+            const int m = m0 + j < p.M ? m0 + j : p.M - 1;
+            v[k][j] = ld_stream(p.psi + base[k] + (size_t)m * slab);
           }
-          val[k] = acc;
-          int x = lc0[i0 + k] + x_off;  // in [-N/2, W + N/2): one wrap each way
-          x += x < 0 ? p.W : 0;
-          x -= x >= p.W ? p.W : 0;
-          xo[k] = x;
-        }
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+#pragma unroll
+          for (int j = 0; j < MC; ++j) {
+            if (m0 + j == 0) {
+              val[k] = v[k][j];
+            } else if (m0 + j < p.M) {
+              val[k].x += v[k][j].x;
+              val[k].y += v[k][j].y;
+            }
+          }
       }
 #pragma unroll
       for (int k = 0; k < K; ++k) {
@@ -893,7 +916,12 @@ int launch_forward(const qem::Params& p, const Layout& L, hipStream_t st) {
   prof_mark(false, 1, st);
   if (int rc = check_launch("forward")) return rc;
   const int nparts = (L.nloss + qem::kLossTiles - 1) / qem::kLossTiles;
-  hipLaunchKernelGGL(qem::k_loss<N>, dim3(nparts), dim3(256), 0, st, p, L.nloss);
+  if (p.M >= 4)
+    hipLaunchKernelGGL((qem::k_loss<N, 4>), dim3(nparts), dim3(256), 0, st, p, L.nloss);
+  else if (p.M >= 2)
+    hipLaunchKernelGGL((qem::k_loss<N, 2>), dim3(nparts), dim3(256), 0, st, p, L.nloss);
+  else
+    hipLaunchKernelGGL((qem::k_loss<N, 1>), dim3(nparts), dim3(256), 0, st, p, L.nloss);
   if (int rc = check_launch("loss")) return rc;
   hipLaunchKernelGGL(qem::k_loss_finalize, dim3(1), dim3(256), 0, st, p, nparts);
   return check_launch("loss_finalize");
@@ -904,7 +932,12 @@ int launch_forward256(const qem::Params& p, const Layout& L, hipStream_t st) {
   if (int rc = qem::launch_chain256_forward(p, st)) return rc;
   prof_mark(false, 1, st);
   const int nparts = (L.nloss + qem::kLossTiles - 1) / qem::kLossTiles;
-  hipLaunchKernelGGL(qem::k_loss<256>, dim3(nparts), dim3(256), 0, st, p, L.nloss);
+  if (p.M >= 4)
+    hipLaunchKernelGGL((qem::k_loss<256, 4>), dim3(nparts), dim3(256), 0, st, p, L.nloss);
+  else if (p.M >= 2)
+    hipLaunchKernelGGL((qem::k_loss<256, 2>), dim3(nparts), dim3(256), 0, st, p, L.nloss);
+  else
+    hipLaunchKernelGGL((qem::k_loss<256, 1>), dim3(nparts), dim3(256), 0, st, p, L.nloss);
   if (int rc = check_launch("loss")) return rc;
   hipLaunchKernelGGL(qem::k_loss_finalize, dim3(1), dim3(256), 0, st, p, nparts);
   return check_launch("loss_finalize");
@@ -931,7 +964,13 @@ template <int N>
 int launch_obj_reduce(const qem::Params& q, hipStream_t st) {
   const size_t row_bytes = (size_t)q.W * sizeof(qem::cf);
   if (row_bytes <= 48 * 1024) {
-    hipLaunchKernelGGL(qem::k_obj_reduce_rows<N>, dim3(q.H, q.s1 - q.s0), dim3(N), row_bytes, st, q);
+    const dim3 g(q.H, q.s1 - q.s0);
+    if (q.M >= 4)
+      hipLaunchKernelGGL((qem::k_obj_reduce_rows<N, 4>), g, dim3(N), row_bytes, st, q);
+    else if (q.M >= 2)
+      hipLaunchKernelGGL((qem::k_obj_reduce_rows<N, 2>), g, dim3(N), row_bytes, st, q);
+    else
+      hipLaunchKernelGGL((qem::k_obj_reduce_rows<N, 1>), g, dim3(N), row_bytes, st, q);
   } else {
     dim3 g((q.W + 31) / 32, (q.H + 31) / 32, q.s1 - q.s0);
     hipLaunchKernelGGL(qem::k_obj_reduce<N>, g, dim3(256), 0, st, q);

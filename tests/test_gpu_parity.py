@@ -77,7 +77,7 @@ def test_fft2_matches_numpy(cuda, n):
 @pytest.mark.parametrize(
     "n,s,m,b,descan",
     [(32, 1, 1, 5, False), (32, 3, 2, 6, True), (64, 1, 1, 7, False), (64, 4, 3, 5, False),
-     (128, 1, 1, 3, False), (128, 3, 2, 4, True), (128, 8, 4, 3, False),
+     (128, 1, 1, 3, False), (128, 3, 2, 4, True), (128, 8, 4, 3, False), (32, 2, 6, 4, False),
      (256, 1, 1, 2, False), (256, 3, 2, 3, True)],
 )
 def test_forward_and_gradients_match_oracle(cuda, n, s, m, b, descan):
