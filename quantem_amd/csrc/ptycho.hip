@@ -34,6 +34,18 @@ template <int N>
 __global__ __launch_bounds__(Fft2<N>::T) void k_probe_spectrum(Params p) {
   __shared__ cf lds[Fft2<N>::LDS_TOTAL];
   using F = Fft2<N>;
+  if ((int)blockIdx.x >= p.M) {
+    // the blocks past the modes fill the shift-ramp table: ramps[b][0][k] = exp(-2 pi i f_k dr_b),
+    // ramps[b][1][k] = exp(-2 pi i f_k dc_b), read by the forward prologue and the adjoint epilogue
+    const int idx = ((int)blockIdx.x - p.M) * F::T + (int)threadIdx.x;
+    if (idx >= p.B * 2 * N) return;
+    const int b = idx / (2 * N), r = idx - b * 2 * N, k = r & (N - 1);
+    int r0, c0;
+    float dr, dc;
+    origin_and_shift<N>(p.pos, b, r0, c0, dr, dc);
+    p.ramps[idx] = expm2pi(fftfreq1<N>(k) * (r < N ? dr : dc));
+    return;
+  }
   F f(threadIdx.x);
   f.init_twiddles(lds, threadIdx.x);
   __syncthreads();
@@ -72,15 +84,15 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
 #pragma unroll
     for (int e = 0; e < PTS; ++e) v[e] = src[f.p1_off(e)];
   } else {
-    // shifted probe: spectrum * ramp / N^2, inverse transform
+    // shifted probe: spectrum * ramp / N^2, inverse transform (row and column ramps from the table)
     const cf* pk = p.pk + (size_t)m * N * N;
-    const cf rc = expm2pi(fftfreq1<N>(f.kc) * dc);
+    const cf* rr = p.ramps + (size_t)b * 2 * N;
+    const cf rc = rr[N + f.kc];
     const float inv_n2 = 1.0f / (float)(N * N);
 #pragma unroll
     for (int e = 0; e < PTS; ++e) {
       const int kr = f.p3_row(e);
-      const cf ramp = cmul(expm2pi(fftfreq1<N>(kr) * dr), rc);
-      v[e] = cscale(cmul(pk[kr * N + f.kc], ramp), inv_n2);
+      v[e] = cscale(cmul(pk[kr * N + f.kc], cmul(rr[kr], rc)), inv_n2);
     }
     f.inv(v, lds);
   }
@@ -360,11 +372,14 @@ __global__ __launch_bounds__(256) void k_adjoint_epilogue(Params p) {
   cf* Xs = p.X + (size_t)item * N * N;
   const cf* pk = p.pk + (size_t)m * N * N;
   if (p.flags & QEM_FLAG_ANALYTIC) return;  // the reference sums g at the probe plane without the shift's adjoint
+  __shared__ cf rtab[2 * N];  // this pattern's row / column ramps
+  for (int i = threadIdx.x; i < 2 * N; i += 256) rtab[i] = p.ramps[(size_t)b * 2 * N + i];
+  __syncthreads();
   double ar = 0.0, ac = 0.0;
   for (int idx = threadIdx.x; idx < N * N; idx += 256) {
     const int kr = idx / N, kc = idx - (idx / N) * N;
     const float fr = fftfreq1<N>(kr), fc = fftfreq1<N>(kc);
-    const cf ramp = cmul(expm2pi(fr * dr), expm2pi(fc * dc));
+    const cf ramp = cmul(rtab[kr], rtab[N + kc]);
     const cf G = Xs[idx];
     Xs[idx] = cmulc(G, ramp);
     const float im = cconjmul(G, cmul(pk[idx], ramp)).y;
@@ -798,7 +813,7 @@ void prof_mark(bool adjoint_kernel, int edge, hipStream_t st) {
 }
 
 struct Layout {
-  size_t psi, ckpt, scratch, X, gI, loss_part, pk, gsum, gpos, gdesc, total;
+  size_t psi, ckpt, scratch, X, gI, loss_part, pk, gsum, gpos, gdesc, ramps, total;
   int nloss;
   int wslots;  // wave slots per item in psi
   int ck, nck; // checkpoint interval (0 = every incident wave is stored) and checkpoints per item
@@ -836,6 +851,8 @@ Layout layout_of(const qem_ptycho_desc& d) {
   o += align_up(BM * 2 * 4);
   L.gdesc = o;
   o += align_up(BM * 2 * 4);
+  L.ramps = o;
+  o += align_up((size_t)d.batch * 2 * d.roi * 8);
   L.total = o;
   return L;
 }
@@ -898,6 +915,7 @@ qem::Params make_params(const qem_ptycho_desc& d, const Layout& L, void* ws) {
   p.loss_part = (double*)(w + L.loss_part);
   p.pk = (qem::cf*)(w + L.pk);
   p.gsum = (qem::cf*)(w + L.gsum);
+  p.ramps = (qem::cf*)(w + L.ramps);
   p.gpos_part = (float*)(w + L.gpos);
   p.gdesc_part = (float*)(w + L.gdesc);
   return p;
@@ -906,7 +924,9 @@ qem::Params make_params(const qem_ptycho_desc& d, const Layout& L, void* ws) {
 template <int N>
 int launch_forward(const qem::Params& p, const Layout& L, hipStream_t st) {
   using F = qem::Fft2<N>;
-  hipLaunchKernelGGL(qem::k_probe_spectrum<N>, dim3(p.M), dim3(F::T), 0, st, p);
+  // M probe-spectrum blocks, then the blocks of the (B, 2, N) shift-ramp table
+  const int ramp_blocks = (p.B * 2 * N + F::T - 1) / F::T;
+  hipLaunchKernelGGL(qem::k_probe_spectrum<N>, dim3(p.M + ramp_blocks), dim3(F::T), 0, st, p);
   if (int rc = check_launch("probe_spectrum")) return rc;
   prof_mark(false, 0, st);
   if (p.descan != nullptr)

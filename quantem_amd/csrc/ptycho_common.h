@@ -34,6 +34,7 @@ struct Params {
   double* loss_part;  // per loss block
   cf* pk;           // (M, N, N)      FFT2 of the probe modes
   cf* gsum;         // (M, N, N)      sum over the batch of probe-gradient spectra
+  cf* ramps;        // (B, 2, N)      sub-pixel shift ramps exp(-2 pi i f dr_b), exp(-2 pi i f dc_b), f = fftfreq(N)
   float* gpos_part;   // (B*M, 2)
   float* gdesc_part;  // (B*M, 2)
   // slice range and wave storage of one launch.  psi holds `wslots` slots per item; slice s of the
