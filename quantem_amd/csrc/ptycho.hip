@@ -121,13 +121,23 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
       dst = p.psi + ((size_t)item * p.wslots + (s - p.s0)) * N * N;
     else if (mode == FWD_CHECKPOINT && s % p.ck == 0)
       dst = p.ckpt + ((size_t)item * p.nck + s / p.ck) * N * N;
+    const bool last_only = !far && s + 1 == p.s1;  // a recomputed segment needs only the incident waves
+    // The object offsets are recomputed every slice: hoisted out of the loop they would hold ~46 VGPRs
+    // through the transforms.  The object loads are issued before the wave stores, because loads and
+    // stores share one vmcnt counter and a load issued after a store waits for that store as well.
+    opaque_offsets<F::RR1, F::RC1>(orow, ocol);
+    cf o[PTS];
+    if (!last_only) {
+#pragma unroll
+      for (int e = 0; e < PTS; ++e) o[e] = obj_at(O, orow[e / F::RC1] + ocol[e % F::RC1]);
+    }
     if (dst != nullptr) {
 #pragma unroll
       for (int e = 0; e < PTS; ++e) st_stream(dst + f.p1_off(e), v[e]);
     }
-    if (!far && s + 1 == p.s1) break;  // a recomputed segment needs only the incident waves
+    if (last_only) break;
 #pragma unroll
-    for (int e = 0; e < PTS; ++e) v[e] = cmul(obj_at(O, orow[e / F::RC1] + ocol[e % F::RC1]), v[e]);
+    for (int e = 0; e < PTS; ++e) v[e] = cmul(o[e], v[e]);
     if (s + 1 < p.s1) {
       f.fwd(v, lds);
       const cf* pr = p.prop + (size_t)s * N * N;
@@ -311,33 +321,44 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
       }
     }
   }
-  // one slice back: object-gradient patch conj(psi_s) g_x (written over psi_s, all loads before
-  // stores), then g_psi = conj(O_s) g_x
-  static_assert(PTS % 16 == 0, "slice-step chunks of 16");
-  auto slice_step = [&](int s) {
+  // One slice back: object-gradient patch conj(psi_s) g_x (written over psi_s), then
+  // g_psi = conj(O_s) g_x, in chunks of CH elements whose psi and object loads are issued before
+  // their stores.  At N = 128 the first PF psi values of the next slice down are loaded before this
+  // slice's propagation transforms (PF = 20 of 32 fills the registers without spilling), so that part
+  // of the next step's HBM latency hides behind the transforms; smaller N keep their occupancy.
+  constexpr int CH = PTS < 16 ? PTS : 16;
+  constexpr int PF = N >= 128 ? (PTS < 20 ? PTS : 20) : 0;
+  cf pre[PF > 0 ? PF : 1];
+  auto slice_step = [&](int s, bool have_pre) {
     const cf* O = p.obj + (size_t)s * HW;
     cf* ps = p.psi + ((size_t)item * p.wslots + (s - p.s0)) * N * N;
-    // chunks of 16 elements: the psi stores of one chunk keep the next chunk's loads from being
-    // hoisted, which bounds the registers in flight; loads and stores share one vmcnt counter, so a
-    // chunk's loads also wait for the previous chunk's stores (fewer, larger chunks wait less)
+    opaque_offsets<F::RR1, F::RC1>(orow, ocol);
 #pragma unroll
-    for (int e0 = 0; e0 < PTS; e0 += 16) {
-      cf t[16], o[16];
+    for (int e0 = 0; e0 < PTS; e0 += CH) {
+      cf t[CH], o[CH];
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        t[i] = ld_stream(ps + f.p1_off(e0 + i));
+      for (int i = 0; i < CH; ++i) {
+        if (e0 + i < PF && have_pre)
+          t[i] = pre[e0 + i];
+        else
+          t[i] = ld_stream(ps + f.p1_off(e0 + i));
         o[i] = obj_at(O, orow[(e0 + i) / F::RC1] + ocol[(e0 + i) % F::RC1]);
       }
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
+      for (int i = 0; i < CH; ++i) {
         st_stream(ps + f.p1_off(e0 + i), cconjmul(t[i], v[e0 + i]));
         v[e0 + i] = cconjmul(o[i], v[e0 + i]);
       }
     }
   };
   for (int s = p.s1 - 1; s >= p.s0; --s) {
-    slice_step(s);
+    slice_step(s, s < p.s1 - 1);
     if (s > 0) {
+      if (PF > 0 && s - 1 >= p.s0) {  // the next slice down belongs to this launch's segment
+        const cf* pn = p.psi + ((size_t)item * p.wslots + (s - 1 - p.s0)) * N * N;
+#pragma unroll
+        for (int i = 0; i < PF; ++i) pre[i] = ld_stream(pn + f.p1_off(i));
+      }
       // g_x <- ifft2(fft2(g_psi) conj(prop_{s-1})), the adjoint of the forward propagation
       f.fwd(v, lds);
       const cf* __restrict__ pr = p.prop + (size_t)(s - 1) * N * N;
