@@ -36,7 +36,7 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_probe_spectrum(Params p) {
   using F = Fft2<N>;
   if ((int)blockIdx.x >= p.M) {
     // the blocks past the modes fill the shift-ramp table: ramps[b][0][k] = exp(-2 pi i f_k dr_b),
-    // ramps[b][1][k] = exp(-2 pi i f_k dc_b), read by the forward prologue and the adjoint epilogue
+    // ramps[b][1][k] = exp(-2 pi i f_k dc_b), read by the forward prologue and the adjoint's last step
     const int idx = ((int)blockIdx.x - p.M) * F::T + (int)threadIdx.x;
     if (idx >= p.B * 2 * N) return;
     const int b = idx / (2 * N), r = idx - b * 2 * N, k = r & (N - 1);
@@ -374,42 +374,31 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
     return;
   }
   f.fwd(v, lds);
-  // G = fft2(g_psi_0), natural order, over X (the epilogue kernel turns it into the probe-gradient
-  // spectrum and the position-gradient partials)
+  // G = fft2(g_psi_0) in natural order.  Over X goes the probe-gradient spectrum conj(ramp_b) G
+  // (G itself for the analytical update: the reference sums g at the probe plane without the
+  // shift's adjoint); the position-gradient partials are (2 pi / N^2) sum f Im(conj(G) P_m ramp_b).
+  if (p.flags & QEM_FLAG_ANALYTIC) {
 #pragma unroll
-  for (int e = 0; e < PTS; ++e) Xs[f.p3_off(e)] = v[e];
-}
-
-// Adjoint epilogue, elementwise on G = fft2(g_psi_0) of one (pattern, mode) per block:
-//   spectrum conj(ramp_b) G (over G), position-gradient partials (2 pi / N^2) sum f Im(conj(G) P_m ramp_b).
-template <int N>
-__global__ __launch_bounds__(256) void k_adjoint_epilogue(Params p) {
-  __shared__ double red[4];
-  const int item = blockIdx.x;
-  const int b = item / p.M, m = item - b * p.M;
-  int r0, c0;
-  float dr, dc;
-  origin_and_shift<N>(p.pos, b, r0, c0, dr, dc);
-  cf* Xs = p.X + (size_t)item * N * N;
+    for (int e = 0; e < PTS; ++e) Xs[f.p3_off(e)] = v[e];
+    return;
+  }
+  const cf* rr = p.ramps + (size_t)b * 2 * N;
   const cf* pk = p.pk + (size_t)m * N * N;
-  if (p.flags & QEM_FLAG_ANALYTIC) return;  // the reference sums g at the probe plane without the shift's adjoint
-  __shared__ cf rtab[2 * N];  // this pattern's row / column ramps
-  for (int i = threadIdx.x; i < 2 * N; i += 256) rtab[i] = p.ramps[(size_t)b * 2 * N + i];
-  __syncthreads();
+  const cf rc = rr[N + f.kc];
+  const float fc = fftfreq1<N>(f.kc);
   double ar = 0.0, ac = 0.0;
-  for (int idx = threadIdx.x; idx < N * N; idx += 256) {
-    const int kr = idx / N, kc = idx - (idx / N) * N;
-    const float fr = fftfreq1<N>(kr), fc = fftfreq1<N>(kc);
-    const cf ramp = cmul(rtab[kr], rtab[N + kc]);
-    const cf G = Xs[idx];
-    Xs[idx] = cmulc(G, ramp);
-    const float im = cconjmul(G, cmul(pk[idx], ramp)).y;
-    ar += (double)(fr * im);
+#pragma unroll
+  for (int e = 0; e < PTS; ++e) {
+    const int kr = f.p3_row(e), off = f.p3_off(e);
+    const cf ramp = cmul(rr[kr], rc);
+    Xs[off] = cmulc(v[e], ramp);
+    const float im = cconjmul(v[e], cmul(pk[off], ramp)).y;
+    ar += (double)(fftfreq1<N>(kr) * im);
     ac += (double)(fc * im);
   }
   if (p.flags & QEM_FLAG_GRAD_POSITIONS) {
-    const double tr = block_sum<256>(ar, red);
-    const double tc = block_sum<256>(ac, red);
+    const double tr = block_sum<F::T>(ar, red);
+    const double tc = block_sum<F::T>(ac, red);
     if (threadIdx.x == 0) {
       p.gpos_part[2 * item] = (float)(2.0 * M_PI * tr / (double)(N * N));
       p.gpos_part[2 * item + 1] = (float)(2.0 * M_PI * tc / (double)(N * N));
@@ -1071,8 +1060,6 @@ int launch_adjoint(const qem::Params& p, hipStream_t st) {
   });
   prof_mark(true, 1, st);
   if (rc) return rc;
-  hipLaunchKernelGGL(qem::k_adjoint_epilogue<N>, dim3(p.B * p.M), dim3(256), 0, st, p);
-  if (int rc = check_launch("adjoint_epilogue")) return rc;
   if (p.ck == 0)
     if (int r = launch_obj_reduce<N>(p, st)) return r;
   hipLaunchKernelGGL(qem::k_probe_reduce<N>, dim3(p.M * N * N / 64), dim3(256), 0, st, p);
