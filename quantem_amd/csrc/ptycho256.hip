@@ -202,14 +202,20 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(1, 1))) void 
     const bool transmit = far || s + 1 < p.s1;  // a recomputed segment needs only its incident waves
     const bool exit_ramp = DESCAN && far && s == p.S - 1;
     auto op = [&](int r, cf(&v)[16]) {
+      // object values first: a load issued after the wave stores would wait for them (one vmcnt)
+      cf o[16];
+      if (transmit) {
+        const unsigned orow = obj_row_off(r0, r, p.H, p.W);
+#pragma unroll
+        for (int e = 0; e < 16; ++e) o[e] = obj_at(O, orow + obj_col_off(q, e, p.W));
+      }
       if (keep != nullptr) {
 #pragma unroll
         for (int e = 0; e < 16; ++e) st_stream(keep + r * N + j + 16 * e, v[e]);
       }
       if (transmit) {
-        const unsigned orow = obj_row_off(r0, r, p.H, p.W);
 #pragma unroll
-        for (int e = 0; e < 16; ++e) v[e] = cmul(obj_at(O, orow + obj_col_off(q, e, p.W)), v[e]);
+        for (int e = 0; e < 16; ++e) v[e] = cmul(o[e], v[e]);
         if (exit_ramp) {
           const float fr = fftfreq1<N>(r);
 #pragma unroll
