@@ -334,21 +334,17 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(1, 1))) void 
     // g_x of slice s in natural order: object-gradient patch conj(psi_s) g over psi_s, g_psi = conj(O_s) g
     row_pass<true, true>((!top && s == p.s1 - 1) ? (const cf*)Xs : (const cf*)Wb, Wb, sm, [&](int r, cf(&v)[16]) {
       const unsigned orow = obj_row_off(r0, r, p.H, p.W);
-      // two halves of 8 elements: all loads of a half before its stores
+      // every load of the row chunk before its stores (a load issued after a store waits for it)
+      cf ps[16], o[16];
 #pragma unroll
-      for (int h = 0; h < 16; h += 8) {
-        cf ps[8], o[8];
+      for (int e = 0; e < 16; ++e) {
+        ps[e] = ld_stream(A + r * N + j + 16 * e);
+        o[e] = obj_at(O, orow + obj_col_off(q, e, p.W));
+      }
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-          ps[i] = ld_stream(A + r * N + j + 16 * (h + i));
-          o[i] = obj_at(O, orow + obj_col_off(q, h + i, p.W));
-        }
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-          const int e = h + i;
-          st_stream(A + r * N + j + 16 * e, cconjmul(ps[i], v[e]));
-          v[e] = cconjmul(o[i], v[e]);
-        }
+      for (int e = 0; e < 16; ++e) {
+        st_stream(A + r * N + j + 16 * e, cconjmul(ps[e], v[e]));
+        v[e] = cconjmul(o[e], v[e]);
       }
     });
     if (s > 0) {
