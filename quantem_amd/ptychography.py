@@ -83,6 +83,20 @@ def workspace_bytes(desc: nat.QemDesc) -> int:
     return int(out.value)
 
 
+_ONES_MASKS: dict = {}
+
+
+def _ones_mask(n: int, device: torch.device) -> torch.Tensor:
+    """The default detector mask (all ones, ``dataset_models.py:177-187``), one read-only tensor per
+    (size, device), so that a batch without a mask costs no fill kernel."""
+    key = (n, device.type, device.index)
+    m = _ONES_MASKS.get(key)
+    if m is None:
+        m = torch.ones((n, n), dtype=torch.float32, device=device)
+        _ONES_MASKS[key] = m
+    return m
+
+
 class MultisliceLoss(torch.autograd.Function):
     """Loss of one batch; backward runs the hand-written adjoint.
 
@@ -122,7 +136,7 @@ class MultisliceLoss(torch.autograd.Function):
             prop_c = None
         tgt_c = _need(targets, "targets", torch.float32, (B, N, N), device)
         if mask is None:
-            mask_c = torch.ones((N, N), dtype=torch.float32, device=device)
+            mask_c = _ones_mask(N, device)
         else:
             mask_c = _need(mask, "mask", torch.float32, (N, N), device)
         desc_c = None if descan is None else _need(descan, "descan", torch.float32, (B, 2), device)
@@ -294,7 +308,7 @@ def analytical_gradients(
     pos_c = _need(positions.detach(), "positions", torch.float32, (B, 2), device)
     prop_c = _need(propagators, "propagators", torch.complex64, (S - 1, N, N), device) if S > 1 else None
     tgt_c = _need(targets, "targets", torch.float32, (B, N, N), device)
-    mask_c = torch.ones((N, N), dtype=torch.float32, device=device) if mask is None else \
+    mask_c = _ones_mask(N, device) if mask is None else \
         _need(mask, "mask", torch.float32, (N, N), device)
     desc_c = None if descan is None else _need(descan.detach(), "descan", torch.float32, (B, 2), device)
     d = _desc(S, M, B, N, H, W, num_gpts, loss_type, mean_intensity, nat.QEM_FLAG_STORE_WAVES)
