@@ -70,3 +70,60 @@ class Adam(torch.optim.Optimizer):
                                            torch.cuda.current_stream(pv.device).cuda_stream)
                 nat.check(rc, "qem_adam_step_f32")
         return loss
+
+
+SCHEDULER_TYPES = ("cyclic", "plateau", "exp", "gamma", "exponential", "linear", "none")
+
+
+def make_scheduler(optimizer: torch.optim.Optimizer, params: dict | None, num_iter: int | None = None):
+    """The learning-rate schedule of one model's optimizer, as the reference builds it
+    (``core/ml/optimizer_mixin.py:128-183``), or None.
+
+    ``params["type"]`` (case-insensitive): "none"; "cyclic" (CyclicLR between base_lr = lr/4 and
+    max_lr = 4 lr, step_size_up 100, mode "triangular2", no momentum cycling); "plateau" (any type
+    starting with "plat" or "reducelronplat": ReduceLROnPlateau, factor 0.5, patience 10, threshold
+    1e-3, min_lr = lr/20, cooldown 50); "exp" / "gamma" / "exponential" (ExponentialLR with ``gamma``,
+    else factor ** (1 / num_iter) with factor 0.01, else 0.9); "linear" (LinearLR from 0.1 to 1.0 over
+    num_iter).  lr is the optimizer's first group's rate; every default can be overridden by a key of
+    the same name (``momentum`` for cycle_momentum).  Step it once per epoch with :func:`step_scheduler`.
+    """
+    if not params:
+        return None
+    sched_type = str(params.get("type", "none")).lower()
+    base = optimizer.param_groups[0]["lr"]
+    sch = torch.optim.lr_scheduler
+    if sched_type == "none":
+        return None
+    if sched_type == "cyclic":
+        up = params.get("step_size_up", 100)
+        return sch.CyclicLR(optimizer, base_lr=params.get("base_lr", base / 4), max_lr=params.get("max_lr", base * 4),
+                            step_size_up=up, step_size_down=params.get("step_size_down", up),
+                            mode=params.get("mode", "triangular2"), cycle_momentum=params.get("momentum", False))
+    if sched_type.startswith(("plat", "reducelronplat")):
+        return sch.ReduceLROnPlateau(optimizer, mode="min", factor=params.get("factor", 0.5),
+                                     patience=params.get("patience", 10), threshold=params.get("threshold", 1e-3),
+                                     min_lr=params.get("min_lr", base / 20), cooldown=params.get("cooldown", 50))
+    if sched_type in ("exp", "gamma", "exponential"):
+        if "gamma" in params:
+            gamma = params["gamma"]
+        elif num_iter is not None:
+            gamma = params.get("factor", 0.01) ** (1.0 / num_iter)
+        else:
+            gamma = 0.9
+        return sch.ExponentialLR(optimizer, gamma=gamma)
+    if sched_type == "linear":
+        return sch.LinearLR(optimizer, start_factor=params.get("start_factor", 0.1),
+                            end_factor=params.get("end_factor", 1.0), total_iters=params.get("total_iters", num_iter))
+    raise ValueError(f"Unknown scheduler type: {sched_type}, expected one of {list(SCHEDULER_TYPES)}")
+
+
+def step_scheduler(scheduler, loss: float | None) -> None:
+    """One epoch's step (``optimizer_mixin.py:195-202``): ReduceLROnPlateau takes the epoch loss (and
+    is skipped without one), the others step unconditionally."""
+    if scheduler is None:
+        return
+    if isinstance(scheduler, torch.optim.lr_scheduler.ReduceLROnPlateau):
+        if loss is not None:
+            scheduler.step(loss)
+    else:
+        scheduler.step()

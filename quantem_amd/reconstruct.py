@@ -21,7 +21,7 @@ import numpy as np
 import torch
 
 from .constraints import object_constraints
-from .optim import Adam
+from .optim import Adam, make_scheduler, step_scheduler
 from .ptychography import analytical_gradients, multislice_loss
 
 
@@ -193,6 +193,7 @@ class Reconstruction:
             _real_view(torch.as_tensor(probe, dtype=torch.complex64, device=dev)).clone())
         self.history: list = []
         self.val_history: list = []
+        self.lr_history: dict = {}
         self.times: list = []
 
     @property
@@ -235,6 +236,7 @@ class Reconstruction:
                     autograd: bool = True, segment_slices: int = 0, dataset_lr: Optional[float] = None,
                     learn_positions: bool = True, learn_descan: bool = True,
                     dataset_constraints: Optional[dict] = None, val_ratio: float = 0.0, val_mode: str = "grid",
+                    schedulers: Optional[dict] = None,
                     callback: Optional[Callable[[int, float, "Reconstruction"], None]] = None) -> list:
         """Runs `num_iters` epochs; returns the per-epoch mean batch loss (consistency + soft constraints).
 
@@ -251,6 +253,12 @@ class Reconstruction:
         Position and descan gradients come from the loss gradient, so they need ``autograd=True``.
         ``val_ratio`` > 0 holds out scan positions (:func:`validation_split`): they take no optimizer
         step, and their mean batch loss after each epoch goes to ``self.val_history``.
+
+        Each model ("object", "probe", "dataset") has its own Adam, as in the reference
+        (``ptychography_opt.py:13-234``); ``schedulers`` maps a model name to the reference's scheduler
+        parameters (:func:`quantem_amd.optim.make_scheduler`, built with ``num_iter = num_iters``),
+        stepped once per epoch with the epoch's mean loss (``ptychography.py:278-279``).  The learning
+        rate of every model after each epoch goes to ``self.lr_history``.
         """
         cons = dict(DATASET_CONSTRAINTS)
         if dataset_constraints:
@@ -266,9 +274,9 @@ class Reconstruction:
         # the object-gradient rescale of the autograd path (ptychography.py:330-332) is applied inside
         # the optimizer step; the analytical update is not rescaled, as in the reference
         scale = self.upsample_factor ** 2 / 2 if autograd else 1.0
-        groups = [{"params": [self._obj], "lr": obj_lr, "grad_scale": scale}]
+        opts = {"object": Adam([{"params": [self._obj], "lr": obj_lr, "grad_scale": scale}])}
         if learn_probe:
-            groups.append({"params": [self._probe], "lr": probe_lr})
+            opts["probe"] = Adam([{"params": [self._probe], "lr": probe_lr}])
         self._probe.requires_grad_(learn_probe)
         dset_params = []
         if learn_dset and learn_descan:
@@ -278,8 +286,14 @@ class Reconstruction:
         for t in (self.positions, self.descan):
             t.requires_grad_(any(t is q for q in dset_params))
         if dset_params:
-            groups.append({"params": dset_params, "lr": float(dataset_lr)})
-        opt = Adam(groups)
+            opts["dataset"] = Adam([{"params": dset_params, "lr": float(dataset_lr)}])
+        sched_cfg = dict(schedulers or {})
+        unknown = set(sched_cfg) - {"object", "probe", "dataset"}
+        if unknown:
+            raise KeyError(f"schedulers for unknown models {sorted(unknown)}; known: object, probe, dataset")
+        scheds = {k: make_scheduler(opts[k], v, num_iters) for k, v in sched_cfg.items() if k in opts}
+        for k in opts:
+            self.lr_history.setdefault(k, [])
         batcher = SimpleBatcher(self.num_gpts, batch_size, rng=seed, val_ratio=val_ratio, val_mode=val_mode)
         tv_weight = float(cons["descan_tv_weight"]) if use_descan else 0.0
 
@@ -298,7 +312,8 @@ class Reconstruction:
             for epoch in range(num_iters):
                 total = torch.zeros((), dtype=torch.float32, device=self.device)
                 for idx in batcher:
-                    opt.zero_grad(set_to_none=True)
+                    for o in opts.values():
+                        o.zero_grad(set_to_none=True)
                     it, pos, desc = batch_inputs(idx)
                     if autograd:
                         loss = multislice_loss(self.obj_constrained, self.probe_constrained, pos, self.propagators,
@@ -317,7 +332,8 @@ class Reconstruction:
                         self._obj.grad = _real_view(og).contiguous()
                         if learn_probe:
                             self._probe.grad = _real_view(pg).contiguous()
-                    opt.step()
+                    for o in opts.values():
+                        o.step()
                     total = total + loss.detach()
                 if batcher.has_validation:
                     vtot = torch.zeros((), dtype=torch.float32, device=self.device)
@@ -331,6 +347,10 @@ class Reconstruction:
                     self.val_history.append(float(vtot.item()) / batcher.val_len())
                 mean_loss = float(total.item()) / len(batcher)  # one host sync per epoch
                 self.history.append(mean_loss)
+                for k, sc in scheds.items():
+                    step_scheduler(sc, mean_loss)
+                for k, o in opts.items():
+                    self.lr_history[k].append(o.param_groups[0]["lr"])
                 self.times.append(time.perf_counter() - t0 - t_cb)
                 if callback is not None:
                     tc = time.perf_counter()
@@ -392,6 +412,8 @@ def main(argv=None) -> int:
     ap.add_argument("--dataset-lr", type=float, default=None,
                     help="learn the scan positions and descan shifts at this Adam rate (default: fixed)")
     ap.add_argument("--val-ratio", type=float, default=0.0, help="fraction of positions held out for validation")
+    ap.add_argument("--schedule", default=None, choices=["cyclic", "plateau", "exp", "linear"],
+                    help="learning-rate schedule of the object and probe optimizers (reference defaults)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default=None, help="write the summary JSON here")
     args = ap.parse_args(argv)
@@ -422,10 +444,12 @@ def main(argv=None) -> int:
     torch.cuda.reset_peak_memory_stats(dev)
     rec.reconstruct(args.epochs, args.batch or spec.batch, learn_probe=args.learn_probe, seed=args.seed,
                     segment_slices=args.segment, dataset_lr=args.dataset_lr, val_ratio=args.val_ratio,
+                    schedulers=({"object": {"type": args.schedule}, "probe": {"type": args.schedule}}
+                                if args.schedule else None),
                     callback=report)
     P = amps.shape[0]
     out = {"workload": spec.name, "patterns": P, "epochs": args.epochs, "batch": args.batch or spec.batch,
-           "segment_slices": args.segment, "learn_probe": args.learn_probe,
+           "segment_slices": args.segment, "learn_probe": args.learn_probe, "schedule": args.schedule,
            "reconstruction_s": round(rec.times[-1], 3),
            "patterns_per_s": round(P * args.epochs / rec.times[-1], 1),
            "final_loss": rec.history[-1], "final_phase_nmse": nmse_hist[-1],

@@ -45,3 +45,46 @@ def test_adam_matches_torch(cuda, shape, dtype, wd, gscale):
         rb = ref.state[b][k]
         rb = torch.view_as_real(rb) if rb.is_complex() else rb
         assert (ra - rb).abs().max() <= 1e-5 * rb.abs().max(), k
+
+
+def test_scheduler_factory_matches_reference_defaults():
+    """make_scheduler builds the reference's schedulers (optimizer_mixin.py:128-183) with its defaults."""
+    from quantem_amd.optim import make_scheduler, step_scheduler
+
+    def opt(lr=0.04):
+        return torch.optim.SGD([torch.nn.Parameter(torch.zeros(3))], lr=lr)
+
+    assert make_scheduler(opt(), None) is None and make_scheduler(opt(), {"type": "none"}) is None
+    o = opt()
+    c = make_scheduler(o, {"type": "cyclic"})
+    assert isinstance(c, torch.optim.lr_scheduler.CyclicLR)
+    assert c.base_lrs == [0.01] and c.max_lrs == [0.16] and c.total_size == 200 and c.mode == "triangular2"
+    o = opt()
+    p = make_scheduler(o, {"type": "Plateau"})
+    assert isinstance(p, torch.optim.lr_scheduler.ReduceLROnPlateau)
+    assert (p.factor, p.patience, p.threshold, p.cooldown, p.min_lrs) == (0.5, 10, 1e-3, 50, [0.002])
+    step_scheduler(p, None)  # no loss: skipped
+    assert p.last_epoch == 0
+    e = make_scheduler(opt(), {"type": "exp"}, num_iter=50)
+    assert e.gamma == pytest.approx(0.01 ** (1 / 50))
+    assert make_scheduler(opt(), {"type": "gamma"}).gamma == 0.9
+    assert make_scheduler(opt(), {"type": "exponential", "gamma": 0.5}).gamma == 0.5
+    o = opt()
+    lin = make_scheduler(o, {"type": "linear"}, num_iter=10)
+    assert isinstance(lin, torch.optim.lr_scheduler.LinearLR) and lin.total_iters == 10
+    assert o.param_groups[0]["lr"] == pytest.approx(0.004)  # starts at 0.1 x
+    for _ in range(10):
+        step_scheduler(lin, 1.0)
+    assert o.param_groups[0]["lr"] == pytest.approx(0.04)
+    with pytest.raises(ValueError):
+        make_scheduler(opt(), {"type": "cosine"})
+
+
+def test_plateau_schedule_halves_after_patience():
+    from quantem_amd.optim import make_scheduler, step_scheduler
+
+    o = torch.optim.SGD([torch.nn.Parameter(torch.zeros(1))], lr=1.0)
+    p = make_scheduler(o, {"type": "plateau", "patience": 2, "cooldown": 0})
+    for loss in (1.0, 1.0, 1.0, 1.0):
+        step_scheduler(p, loss)
+    assert o.param_groups[0]["lr"] == pytest.approx(0.5)  # 3 epochs without improvement after the best

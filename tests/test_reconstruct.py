@@ -242,3 +242,18 @@ def test_descan_refinement_and_validation_loss(cuda):
     assert np.allclose(rec.positions.cpu().numpy(), prob["positions"])  # positions not learned
     assert hist[-1] < hist[0]
     assert err < 0.5 * err0
+
+
+@pytest.mark.gpu
+def test_schedulers_drive_the_per_model_rates(cuda):
+    """An exponential schedule on the object and a plateau schedule on the probe act per epoch."""
+    spec, prob, amps, mean_I = _small_problem(cuda)
+    rec = rc.Reconstruction(prob["positions"], amps, None, prob["probe"], (1,) + spec.obj_shape, mean_I,
+                            upsample_factor=spec.scan_step, device=cuda)
+    hist = rec.reconstruct(3, 48, obj_lr=2e-2, probe_lr=1e-3, seed=0,
+                           schedulers={"object": {"type": "exp", "gamma": 0.5}, "probe": {"type": "plateau"}})
+    assert np.isfinite(hist).all()
+    assert rec.lr_history["object"] == pytest.approx([1e-2, 5e-3, 2.5e-3])
+    assert rec.lr_history["probe"] == pytest.approx([1e-3] * 3)
+    with pytest.raises(KeyError):
+        rec.reconstruct(1, 48, schedulers={"positions": {"type": "exp"}})
