@@ -109,6 +109,64 @@ DATASET_CONSTRAINTS = {
 }
 
 
+# the object and probe soft-constraint weights and their defaults (object_models.py:274-287,
+# probe_models.py:431-436)
+OBJECT_SOFT_CONSTRAINTS = {"tv_weight_z": 0.0, "tv_weight_xy": 0.0, "surface_zero_weight": 0.0}
+PROBE_SOFT_CONSTRAINTS = {"tv_weight": 0.0}
+
+
+def _tv(array: torch.Tensor, w_z: float, w_xy: float) -> torch.Tensor:
+    """sum over axes with a positive weight of w * mean |diff| (axis 0 of a 3-D array takes w_z, the
+    others w_xy), divided by the number of such axes (object_models.py:402-415)."""
+    loss = torch.zeros((), dtype=torch.float32, device=array.device)
+    n = 0
+    for dim in range(array.ndim):
+        w = w_z if (dim == 0 and array.ndim == 3) else w_xy
+        if w > 0:
+            n += 1
+            loss = loss + w * torch.mean(torch.abs(array.diff(dim=dim)))
+    return loss / n if n else loss
+
+
+def object_soft_loss(obj: torch.Tensor, obj_type: str = "complex", tv_weight_z: float = 0.0,
+                     tv_weight_xy: float = 0.0, surface_zero_weight: float = 0.0) -> torch.Tensor:
+    """ObjectConstraints.apply_soft_constraints (object_models.py:341-451) of the hard-constrained object
+    ((S, H, W) complex transmission, or the real potential for obj_type "potential"): total variation of
+    the phase (and, for "complex", the amplitude), without the z term for one slice, plus the
+    surface-zero term (first and last slice pulled to zero phase / unit amplitude, 3 slices or more)."""
+    loss = torch.zeros((), dtype=torch.float32, device=obj.device)
+    w_z = tv_weight_z if obj.shape[0] > 1 else 0.0
+    if w_z > 0 or tv_weight_xy > 0:
+        if obj.is_complex():
+            loss = loss + _tv(obj.angle(), w_z, tv_weight_xy)
+            if obj_type == "complex":
+                loss = loss + _tv(obj.abs(), w_z, tv_weight_xy)
+        else:
+            loss = loss + _tv(obj, w_z, tv_weight_xy)
+    if surface_zero_weight != 0 and obj.shape[0] >= 3:
+        if obj.is_complex():
+            ph = obj.angle().abs()
+            if obj_type == "complex":
+                amp = obj.abs()
+                loss = loss + surface_zero_weight * (torch.mean(1.0 - amp[0]) + torch.mean(1.0 - amp[-1]))
+            loss = loss + surface_zero_weight * (torch.mean(torch.abs(ph[0] - ph[0].mean())) +
+                                                 torch.mean(torch.abs(ph[-1] - ph[-1].mean())))
+        else:
+            loss = loss + surface_zero_weight * (torch.mean(torch.abs(obj[0])) + torch.mean(torch.abs(obj[-1])))
+    return loss
+
+
+def probe_soft_loss(probe: torch.Tensor, tv_weight: float = 0.0) -> torch.Tensor:
+    """ProbeConstraints.apply_soft_constraints (probe_models.py:441-467): tv_weight * (mean |diff| along
+    columns + along rows) of the complex probe modes."""
+    tv = torch.zeros((), dtype=torch.float32, device=probe.device)
+    if tv_weight == 0:
+        return tv
+    for dim in (-1, -2):
+        tv = tv + torch.mean(torch.abs(torch.diff(probe, dim=dim)))
+    return tv_weight * tv
+
+
 def object_hard_constraints(obj: torch.Tensor, obj_type: str = "complex") -> torch.Tensor:
     """ObjectConstraints.apply_hard_constraints for the default constraint set (object_models.py:289-339),
     in torch (the loop uses :func:`quantem_amd.constraints.object_constraints` on the GPU)."""
@@ -236,7 +294,8 @@ class Reconstruction:
                     autograd: bool = True, segment_slices: int = 0, dataset_lr: Optional[float] = None,
                     learn_positions: bool = True, learn_descan: bool = True,
                     dataset_constraints: Optional[dict] = None, val_ratio: float = 0.0, val_mode: str = "grid",
-                    schedulers: Optional[dict] = None,
+                    schedulers: Optional[dict] = None, object_constraints: Optional[dict] = None,
+                    probe_constraints: Optional[dict] = None,
                     callback: Optional[Callable[[int, float, "Reconstruction"], None]] = None) -> list:
         """Runs `num_iters` epochs; returns the per-epoch mean batch loss (consistency + soft constraints).
 
@@ -259,13 +318,25 @@ class Reconstruction:
         parameters (:func:`quantem_amd.optim.make_scheduler`, built with ``num_iter = num_iters``),
         stepped once per epoch with the epoch's mean loss (``ptychography.py:278-279``).  The learning
         rate of every model after each epoch goes to ``self.lr_history``.
+
+        ``object_constraints`` / ``probe_constraints`` override the soft-constraint weights
+        (:data:`OBJECT_SOFT_CONSTRAINTS`, :data:`PROBE_SOFT_CONSTRAINTS`); their losses join every
+        batch loss (``ptychography.py:231-232``), so with ``autograd=False`` they are reported but do not
+        act, as in the reference.
         """
-        cons = dict(DATASET_CONSTRAINTS)
-        if dataset_constraints:
-            unknown = set(dataset_constraints) - set(cons)
-            if unknown:
-                raise KeyError(f"unknown dataset constraints {sorted(unknown)}; known: {sorted(cons)}")
-            cons.update(dataset_constraints)
+        def merged(defaults, given, what):
+            out = dict(defaults)
+            if given:
+                unknown = set(given) - set(out)
+                if unknown:
+                    raise KeyError(f"unknown {what} constraints {sorted(unknown)}; known: {sorted(out)}")
+                out.update(given)
+            return out
+
+        cons = merged(DATASET_CONSTRAINTS, dataset_constraints, "dataset")
+        ocons = merged(OBJECT_SOFT_CONSTRAINTS, object_constraints, "object")
+        pcons = merged(PROBE_SOFT_CONSTRAINTS, probe_constraints, "probe")
+        obj_soft = any(v != 0 for v in ocons.values())
         learn_dset = dataset_lr is not None and (learn_positions or learn_descan)
         if learn_dset and not autograd:
             raise ValueError("learning positions or descan needs autograd=True (the analytical update gives "
@@ -297,6 +368,19 @@ class Reconstruction:
         batcher = SimpleBatcher(self.num_gpts, batch_size, rng=seed, val_ratio=val_ratio, val_mode=val_mode)
         tv_weight = float(cons["descan_tv_weight"]) if use_descan else 0.0
 
+        def soft_loss(oc, pc):
+            """the batch's soft-constraint loss (object, probe, descan), or None when every weight is 0"""
+            terms = []
+            if obj_soft:
+                o = torch.clamp(self.obj, min=0.0) if self.obj_type == "potential" else oc
+                terms.append(object_soft_loss(o, self.obj_type, ocons["tv_weight_z"], ocons["tv_weight_xy"],
+                                              ocons["surface_zero_weight"]))
+            if pcons["tv_weight"] != 0:
+                terms.append(probe_soft_loss(pc, pcons["tv_weight"]))
+            if tv_weight > 0:
+                terms.append(self.descan_tv_loss(tv_weight))
+            return sum(terms[1:], terms[0]) if terms else None
+
         def batch_inputs(idx):
             it = torch.from_numpy(idx).to(self.device)
             self.apply_dataset_constraints(cons)  # every batch, as the dataset's forward does
@@ -316,19 +400,24 @@ class Reconstruction:
                         o.zero_grad(set_to_none=True)
                     it, pos, desc = batch_inputs(idx)
                     if autograd:
-                        loss = multislice_loss(self.obj_constrained, self.probe_constrained, pos, self.propagators,
-                                               self.targets[it], self.mask, desc, num_gpts=self.num_gpts,
-                                               mean_intensity=self.mean_intensity, loss_type=loss_type,
-                                               segment_slices=segment_slices)
-                        if tv_weight > 0:
-                            loss = loss + self.descan_tv_loss(tv_weight)
+                        oc, pc = self.obj_constrained, self.probe_constrained
+                        loss = multislice_loss(oc, pc, pos, self.propagators, self.targets[it], self.mask, desc,
+                                               num_gpts=self.num_gpts, mean_intensity=self.mean_intensity,
+                                               loss_type=loss_type, segment_slices=segment_slices)
+                        soft = soft_loss(oc, pc)
+                        if soft is not None:
+                            loss = loss + soft
                         loss.backward()
                     else:
                         with torch.no_grad():
+                            oc, pc = self.obj_constrained, self.probe_constrained
                             loss, og, pg = analytical_gradients(
-                                self.obj_constrained, self.probe_constrained, pos, self.propagators,
-                                self.targets[it], self.mask, None, num_gpts=self.num_gpts,
-                                mean_intensity=self.mean_intensity, loss_type=loss_type, obj_type=self.obj_type)
+                                oc, pc, pos, self.propagators, self.targets[it], self.mask, None,
+                                num_gpts=self.num_gpts, mean_intensity=self.mean_intensity, loss_type=loss_type,
+                                obj_type=self.obj_type)
+                            soft = soft_loss(oc, pc)
+                            if soft is not None:
+                                loss = loss + soft
                         self._obj.grad = _real_view(og).contiguous()
                         if learn_probe:
                             self._probe.grad = _real_view(pg).contiguous()

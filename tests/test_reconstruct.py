@@ -257,3 +257,59 @@ def test_schedulers_drive_the_per_model_rates(cuda):
     assert rec.lr_history["probe"] == pytest.approx([1e-3] * 3)
     with pytest.raises(KeyError):
         rec.reconstruct(1, 48, schedulers={"positions": {"type": "exp"}})
+
+
+def test_soft_constraint_losses_known_answers():
+    """TV / surface-zero losses of the reference's object and probe soft constraints on hand-sized arrays
+    (object_models.py:341-451, probe_models.py:441-467)."""
+    # real (potential) object, 3 slices of 2 x 2: z diffs, row diffs, column diffs
+    v = torch.tensor([[[0., 1.], [2., 3.]], [[1., 1.], [2., 5.]], [[0., 0.], [0., 0.]]])
+    mz = torch.mean(torch.abs(v.diff(dim=0)))
+    mr = torch.mean(torch.abs(v.diff(dim=1)))
+    mc = torch.mean(torch.abs(v.diff(dim=2)))
+    got = rc.object_soft_loss(v, "potential", tv_weight_z=0.5, tv_weight_xy=0.25)
+    assert float(got) == pytest.approx(float(0.5 * mz + 0.25 * mr + 0.25 * mc) / 3)
+    # only the xy weight: divided by the two axes it covers
+    got = rc.object_soft_loss(v, "potential", tv_weight_xy=1.0)
+    assert float(got) == pytest.approx(float(mr + mc) / 2)
+    # a single slice ignores the z weight
+    assert float(rc.object_soft_loss(v[:1], "potential", tv_weight_z=3.0)) == 0.0
+    # surface zero on the potential: mean |first| + mean |last| slice
+    assert float(rc.object_soft_loss(v, "potential", surface_zero_weight=2.0)) == pytest.approx(2.0 * 1.5)
+    assert float(rc.object_soft_loss(v[:2], "potential", surface_zero_weight=2.0)) == 0.0
+    # complex: phase TV plus amplitude TV; pure phase: phase only
+    ph = torch.tensor([[[0.0, 0.3], [0.1, 0.2]]])
+    amp = torch.tensor([[[1.0, 0.5], [0.8, 0.8]]])
+    o = (amp * torch.exp(1j * ph)).to(torch.complex64)
+    tv_ph = (torch.mean(torch.abs(ph.diff(dim=1))) + torch.mean(torch.abs(ph.diff(dim=2)))) / 2
+    tv_amp = (torch.mean(torch.abs(amp.diff(dim=1))) + torch.mean(torch.abs(amp.diff(dim=2)))) / 2
+    assert float(rc.object_soft_loss(o, "complex", tv_weight_xy=1.0)) == pytest.approx(float(tv_ph + tv_amp), rel=1e-5)
+    assert float(rc.object_soft_loss(o, "pure_phase", tv_weight_xy=1.0)) == pytest.approx(float(tv_ph), rel=1e-5)
+    # probe TV: weight * (mean |column diff| + mean |row diff|) of the complex modes
+    p = torch.tensor([[[0, 1j], [1, 1 + 1j]]], dtype=torch.complex64)
+    assert float(rc.probe_soft_loss(p, 0.5)) == pytest.approx(0.5 * (1.0 + 1.0))
+    assert float(rc.probe_soft_loss(p, 0.0)) == 0.0
+    with pytest.raises(KeyError):
+        rc.Reconstruction(np.zeros((2, 2), np.float32), np.zeros((2, 32, 32), np.float32), None,
+                          np.ones((1, 32, 32), np.complex64), (1, 64, 64), 1.0,
+                          device="cpu").reconstruct(1, 2, object_constraints={"tv_weight": 1.0})
+
+
+@pytest.mark.gpu
+def test_object_tv_constraint_smooths_the_reconstruction(cuda):
+    """A positive xy TV weight adds to the loss and leaves a smoother phase than the same run without it."""
+    spec, prob, amps, mean_I = _small_problem(cuda)
+
+    def run(w):
+        rec = rc.Reconstruction(prob["positions"], amps, None, prob["probe"], (1,) + spec.obj_shape, mean_I,
+                                upsample_factor=spec.scan_step, device=cuda)
+        hist = rec.reconstruct(6, 32, obj_lr=2e-2, learn_probe=False, seed=0,
+                               object_constraints={"tv_weight_xy": w})
+        ph = rec.obj_constrained.detach().angle()
+        return hist, float(torch.mean(torch.abs(ph.diff(dim=1))) + torch.mean(torch.abs(ph.diff(dim=2))))
+
+    h0, tv0 = run(0.0)
+    h1, tv1 = run(5.0)
+    print(f"phase TV without / with the constraint: {tv0:.4g} / {tv1:.4g}", flush=True)
+    assert np.isfinite(h1).all()
+    assert tv1 < tv0
