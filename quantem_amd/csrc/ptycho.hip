@@ -60,6 +60,36 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_probe_spectrum(Params p) {
   for (int e = 0; e < F::PTS; ++e) dst[f.p3_off(e)] = v[e];
 }
 
+// Byte offsets of a thread's object pixels in one slice (layout P1: register e at patch row
+// (e / RC1) * NRP + rp, column (e % RC1) * RC2 + c0, corner-centred).  The column of register
+// e is qq + RC2 ((e % RC1 + RC1/2) % RC1) with qq the object column of patch column c0 - N/2, wrapped
+// once (W >= N); only the RR1 row offsets and qq stay in registers.
+template <int N>
+struct ObjOffsets {
+  using F = Fft2<N>;
+  unsigned orow[F::RR1];
+  int qq, W;
+  __device__ __forceinline__ ObjOffsets(const F& f, int r0, int c0, int H, int W_) : W(W_) {
+#pragma unroll
+    for (int i = 0; i < F::RR1; ++i) {
+      const int r = i * F::NRP + f.rp;
+      orow[i] = (unsigned)(pmod(r0 + (r < N / 2 ? r : r - N), H) * W_) * 8u;
+    }
+    qq = pmod(c0 - N / 2 + f.c0, W_);
+  }
+  __device__ __forceinline__ unsigned at(int e) const {
+    int x = qq + F::RC2 * ((e % F::RC1 + F::RC1 / 2) % F::RC1);
+    x -= x >= W ? W : 0;
+    return orow[e / F::RC1] + (unsigned)x * 8u;
+  }
+  // Keeps offsets derived from these values inside a slice loop from being hoisted out of it.
+  __device__ __forceinline__ void opaque() {
+#pragma unroll
+    for (int i = 0; i < F::RR1; ++i) __asm__ volatile("" : "+v"(orow[i]));
+    __asm__ volatile("" : "+v"(qq));
+  }
+};
+
 // ---------------------------------------------------------------------------------------------
 // Forward multislice for one (pattern, mode): psi_0 = ifft2(fft2(P_m) * ramp_b);
 // for s: x_s = O_s[patch] * psi_s; psi_{s+1} = ifft2(fft2(x_s) * prop_s); X = fft2_ortho(x_{S-1}).
@@ -97,18 +127,7 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
     f.inv(v, lds);
   }
 
-  // byte offsets (32-bit, from a uniform slice base) of this thread's object rows / cols
-  unsigned orow[F::RR1], ocol[F::RC1];
-#pragma unroll
-  for (int i = 0; i < F::RR1; ++i) {
-    const int r = i * F::NRP + f.rp;
-    orow[i] = (unsigned)(pmod(r0 + (r < N / 2 ? r : r - N), p.H) * p.W) * 8u;
-  }
-#pragma unroll
-  for (int i = 0; i < F::RC1; ++i) {
-    const int c = i * F::RC2 + f.c0;
-    ocol[i] = (unsigned)pmod(c0 + (c < N / 2 ? c : c - N), p.W) * 8u;
-  }
+  ObjOffsets<N> oo(f, r0, c0, p.H, p.W);  // byte offsets of this thread's object pixels
 
   const int mode = p.fwd_mode;
   const bool far = mode != FWD_RECOMPUTE;
@@ -125,11 +144,11 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
     // The object offsets are recomputed every slice: hoisted out of the loop they would hold ~46 VGPRs
     // through the transforms.  The object loads are issued before the wave stores, because loads and
     // stores share one vmcnt counter and a load issued after a store waits for that store as well.
-    opaque_offsets<F::RR1, F::RC1>(orow, ocol);
+    oo.opaque();
     cf o[PTS];
     if (!last_only) {
 #pragma unroll
-      for (int e = 0; e < PTS; ++e) o[e] = obj_at(O, orow[e / F::RC1] + ocol[e % F::RC1]);
+      for (int e = 0; e < PTS; ++e) o[e] = obj_at(O, oo.at(e));
     }
     if (dst != nullptr) {
 #pragma unroll
@@ -277,18 +296,7 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
   }
   if (top) f.inv(v, lds);
 
-  // byte offsets (32-bit, from a uniform slice base) of this thread's object rows / cols
-  unsigned orow[F::RR1], ocol[F::RC1];
-#pragma unroll
-  for (int i = 0; i < F::RR1; ++i) {
-    const int r = i * F::NRP + f.rp;
-    orow[i] = (unsigned)(pmod(r0 + (r < N / 2 ? r : r - N), p.H) * p.W) * 8u;
-  }
-#pragma unroll
-  for (int i = 0; i < F::RC1; ++i) {
-    const int c = i * F::RC2 + f.c0;
-    ocol[i] = (unsigned)pmod(c0 + (c < N / 2 ? c : c - N), p.W) * 8u;
-  }
+  ObjOffsets<N> oo(f, r0, c0, p.H, p.W);  // byte offsets of this thread's object pixels
 
   const size_t HW = (size_t)p.H * p.W;
   const bool want_desc = DESCAN && (p.flags & QEM_FLAG_GRAD_DESCAN);
@@ -305,7 +313,7 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
       const float fr = fftfreq1<N>(row), fc = fftfreq1<N>(col);
       const cf ramp = cmul(expm2pi(fr * ddr), expm2pi(fc * ddc));
       if (want_desc) {
-        const cf xo = cmul(cmul(obj_at(O, orow[e / F::RC1] + ocol[e % F::RC1]), ps[f.p1_off(e)]), ramp);
+        const cf xo = cmul(cmul(obj_at(O, oo.at(e)), ps[f.p1_off(e)]), ramp);
         const float im = cconjmul(v[e], xo).y;
         ar += (double)(fr * im);
         ac += (double)(fc * im);
@@ -323,16 +331,16 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
   }
   // One slice back: object-gradient patch conj(psi_s) g_x (written over psi_s), then
   // g_psi = conj(O_s) g_x, in chunks of CH elements whose psi and object loads are issued before
-  // their stores.  At N = 128 the first PF psi values of the next slice down are loaded before this
-  // slice's propagation transforms (PF = 20 of 32 fills the registers without spilling), so that part
-  // of the next step's HBM latency hides behind the transforms; smaller N keep their occupancy.
+  // their stores.  At N = 128 the next slice's psi values (all 32; 20 in the descan variant, whose
+  // extra state would otherwise spill) are loaded before this slice's propagation transforms, so that
+  // the next step's HBM latency hides behind the transforms; smaller N keep their occupancy instead.
   constexpr int CH = PTS < 16 ? PTS : 16;
-  constexpr int PF = N >= 128 ? (PTS < 20 ? PTS : 20) : 0;
+  constexpr int PF = N >= 128 ? (DESCAN ? 20 : PTS) : 0;
   cf pre[PF > 0 ? PF : 1];
   auto slice_step = [&](int s, bool have_pre) {
     const cf* O = p.obj + (size_t)s * HW;
     cf* ps = p.psi + ((size_t)item * p.wslots + (s - p.s0)) * N * N;
-    opaque_offsets<F::RR1, F::RC1>(orow, ocol);
+    oo.opaque();
 #pragma unroll
     for (int e0 = 0; e0 < PTS; e0 += CH) {
       cf t[CH], o[CH];
@@ -342,7 +350,7 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
           t[i] = pre[e0 + i];
         else
           t[i] = ld_stream(ps + f.p1_off(e0 + i));
-        o[i] = obj_at(O, orow[(e0 + i) / F::RC1] + ocol[(e0 + i) % F::RC1]);
+        o[i] = obj_at(O, oo.at(e0 + i));
       }
 #pragma unroll
       for (int i = 0; i < CH; ++i) {

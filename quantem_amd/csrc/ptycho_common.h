@@ -74,17 +74,6 @@ __device__ __forceinline__ cf expm2pi(float x) {
   return mk(c, s);
 }
 
-// Makes the per-thread object row / column offsets opaque to the optimiser at this point (an empty asm
-// statement that "modifies" them), so that offsets derived from them inside a slice loop are
-// recomputed each slice instead of being hoisted and kept live across the transforms.
-template <int NR, int NC>
-__device__ __forceinline__ void opaque_offsets(unsigned (&orow)[NR], unsigned (&ocol)[NC]) {
-#pragma unroll
-  for (int i = 0; i < NR; ++i) __asm__ volatile("" : "+v"(orow[i]));
-#pragma unroll
-  for (int i = 0; i < NC; ++i) __asm__ volatile("" : "+v"(ocol[i]));
-}
-
 // Maps a linear block index to a work item so that consecutive items land on the same XCD
 // (blocks are dispatched round-robin over the 8 XCDs; each XCD has its own L2).
 __device__ __forceinline__ int xcd_item(int blk, int nblk) {
