@@ -89,20 +89,46 @@ __device__ __forceinline__ void fft256(cf (&v)[16], Smem& sm) {
 
 // Row pass over the 256 rows of src (CH rows per chunk): v <- row (columns j + 16 e); inverse row DFT
 // if PRE_INV; op(r, v) on the natural-order row; forward row DFT if POST_FWD; row -> dst (if not null).
-template <bool PRE_INV, bool POST_FWD, typename Op>
+// EARLY_NEXT: each chunk issues the next chunk's loads before its own stores (a load issued after a
+// store waits for it: one vmcnt counter) and carries them in registers (measured faster for the
+// adjoint's passes, slower for the forward's, whose registers it pushes into AGPRs).
+template <bool PRE_INV, bool POST_FWD, bool EARLY_NEXT = false, typename Op>
 __device__ __forceinline__ void row_pass(const cf* src, cf* dst, Smem& sm, Op&& op) {
   const int j = row_j();
-  for (int r0 = 0; r0 < N; r0 += CH) {
-    const int r = r0 + row_vec();
+  if constexpr (EARLY_NEXT) {
     cf v[16];
 #pragma unroll
-    for (int e = 0; e < 16; ++e) v[e] = src[r * N + j + 16 * e];
-    if constexpr (PRE_INV) fft256<true, false>(v, sm);
-    op(r, v);
-    if constexpr (POST_FWD) fft256<false, false>(v, sm);
-    if (dst != nullptr) {
+    for (int e = 0; e < 16; ++e) v[e] = src[row_vec() * N + j + 16 * e];
+    for (int r0 = 0; r0 < N; r0 += CH) {
+      const int r = r0 + row_vec();
+      if constexpr (PRE_INV) fft256<true, false>(v, sm);
+      op(r, v);
+      if constexpr (POST_FWD) fft256<false, false>(v, sm);
+      cf nxt[16];
+      if (r0 + CH < N) {  // the chunks touch disjoint rows
 #pragma unroll
-      for (int e = 0; e < 16; ++e) dst[r * N + j + 16 * e] = v[e];
+        for (int e = 0; e < 16; ++e) nxt[e] = src[(r + CH) * N + j + 16 * e];
+      }
+      if (dst != nullptr) {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) dst[r * N + j + 16 * e] = v[e];
+      }
+#pragma unroll
+      for (int e = 0; e < 16; ++e) v[e] = nxt[e];
+    }
+  } else {
+    for (int r0 = 0; r0 < N; r0 += CH) {
+      const int r = r0 + row_vec();
+      cf v[16];
+#pragma unroll
+      for (int e = 0; e < 16; ++e) v[e] = src[r * N + j + 16 * e];
+      if constexpr (PRE_INV) fft256<true, false>(v, sm);
+      op(r, v);
+      if constexpr (POST_FWD) fft256<false, false>(v, sm);
+      if (dst != nullptr) {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) dst[r * N + j + 16 * e] = v[e];
+      }
     }
   }
   __syncthreads();  // dst complete and visible to the whole workgroup
@@ -110,17 +136,36 @@ __device__ __forceinline__ void row_pass(const cf* src, cf* dst, Smem& sm, Op&& 
 
 // Column pass over the 256 columns of src (CH columns per chunk): v <- column c (rows j + 16 e);
 // op(c, v) (which runs the column DFTs it needs); column -> dst.
-template <typename Op>
+template <bool EARLY_NEXT = false, typename Op>
 __device__ __forceinline__ void col_pass(const cf* src, cf* dst, Smem& sm, Op&& op) {
   const int j = col_j();
-  for (int c0 = 0; c0 < N; c0 += CH) {
-    const int c = c0 + col_vec();
+  if constexpr (EARLY_NEXT) {
     cf v[16];
 #pragma unroll
-    for (int e = 0; e < 16; ++e) v[e] = src[(j + 16 * e) * N + c];
-    op(c, v);
+    for (int e = 0; e < 16; ++e) v[e] = src[(j + 16 * e) * N + col_vec()];
+    for (int c0 = 0; c0 < N; c0 += CH) {
+      const int c = c0 + col_vec();
+      op(c, v);
+      cf nxt[16];
+      if (c0 + CH < N) {  // the chunks touch disjoint columns
 #pragma unroll
-    for (int e = 0; e < 16; ++e) dst[(j + 16 * e) * N + c] = v[e];
+        for (int e = 0; e < 16; ++e) nxt[e] = src[(j + 16 * e) * N + c + CH];
+      }
+#pragma unroll
+      for (int e = 0; e < 16; ++e) dst[(j + 16 * e) * N + c] = v[e];
+#pragma unroll
+      for (int e = 0; e < 16; ++e) v[e] = nxt[e];
+    }
+  } else {
+    for (int c0 = 0; c0 < N; c0 += CH) {
+      const int c = c0 + col_vec();
+      cf v[16];
+#pragma unroll
+      for (int e = 0; e < 16; ++e) v[e] = src[(j + 16 * e) * N + c];
+      op(c, v);
+#pragma unroll
+      for (int e = 0; e < 16; ++e) dst[(j + 16 * e) * N + c] = v[e];
+    }
   }
   __syncthreads();
 }
@@ -332,7 +377,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(1, 1))) void 
     const cf* O = p.obj + (size_t)s * HW;
     cf* A = p.psi + ((size_t)item * p.wslots + (s - p.s0)) * N2;
     // g_x of slice s in natural order: object-gradient patch conj(psi_s) g over psi_s, g_psi = conj(O_s) g
-    row_pass<true, true>((!top && s == p.s1 - 1) ? (const cf*)Xs : (const cf*)Wb, Wb, sm, [&](int r, cf(&v)[16]) {
+    row_pass<true, true, true>((!top && s == p.s1 - 1) ? (const cf*)Xs : (const cf*)Wb, Wb, sm, [&](int r, cf(&v)[16]) {
       const unsigned orow = obj_row_off(r0, r, p.H, p.W);
       // every load of the row chunk before its stores (a load issued after a store waits for it)
       cf ps[16], o[16];
@@ -350,7 +395,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(1, 1))) void 
     if (s > 0) {
       // g_x <- ifft2(fft2(g_psi) conj(prop_{s-1})); at a segment bottom it goes to X for the segment below
       const cf* pr = p.prop + (size_t)(s - 1) * N2;
-      col_pass(Wb, s == p.s0 ? Xs : Wb, sm, [&](int kc, cf(&v)[16]) {
+      col_pass<true>(Wb, s == p.s0 ? Xs : Wb, sm, [&](int kc, cf(&v)[16]) {
         fft256<false, true>(v, sm);
         const int jj = col_j();
 #pragma unroll
