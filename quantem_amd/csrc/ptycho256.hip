@@ -90,8 +90,9 @@ __device__ __forceinline__ void fft256(cf (&v)[16], Smem& sm) {
 // Row pass over the 256 rows of src (CH rows per chunk): v <- row (columns j + 16 e); inverse row DFT
 // if PRE_INV; op(r, v) on the natural-order row; forward row DFT if POST_FWD; row -> dst (if not null).
 // EARLY_NEXT: each chunk issues the next chunk's loads before its own stores (a load issued after a
-// store waits for it: one vmcnt counter) and carries them in registers (measured faster for the
-// adjoint's passes, slower for the forward's, whose registers it pushes into AGPRs).
+// store waits for it: one vmcnt counter) and carries them in registers.  Measured faster for both
+// passes of the adjoint and for the forward's row pass; with it in the forward's column pass as well
+// the forward's registers spill into AGPRs and it runs slower.
 template <bool PRE_INV, bool POST_FWD, bool EARLY_NEXT = false, typename Op>
 __device__ __forceinline__ void row_pass(const cf* src, cf* dst, Smem& sm, Op&& op) {
   const int j = row_j();
@@ -277,7 +278,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(1, 1))) void 
       else
         row_pass<false, false>(ck, nullptr, sm, op);
     } else if (transmit) {
-      row_pass<true, true>(Wb, Wb, sm, op);
+      row_pass<true, true, true>(Wb, Wb, sm, op);
     } else {
       row_pass<true, false>(Wb, nullptr, sm, op);
     }
