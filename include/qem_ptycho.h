@@ -64,7 +64,7 @@ enum {
 typedef struct qem_ptycho_desc {
   int32_t num_slices;      /* S  >= 1 */
   int32_t num_modes;       /* M  >= 1 */
-  int32_t batch;           /* B  >= 1, patterns in this call */
+  int32_t batch;           /* B  >= 1, patterns in this call; B * M * roi^2 < 2^31 */
   int32_t roi;             /* N  in {32, 64, 128, 256} */
   int32_t obj_rows;        /* Ho >= N */
   int32_t obj_cols;        /* Wo >= N */

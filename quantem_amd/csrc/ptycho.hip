@@ -888,6 +888,9 @@ int check_desc(const qem_ptycho_desc* d) {
   if ((size_t)d->num_slices * d->obj_rows * d->obj_cols >= ((size_t)1 << 31) ||
       (size_t)d->obj_rows * d->obj_cols > ((size_t)1 << 28))
     return fail(QEM_ERR_ARG, "object has too many elements (at most 2^28 per slice, 2^31 in total)");
+  if ((size_t)d->batch * d->num_modes * d->roi * d->roi >= ((size_t)1 << 31))
+    return fail(QEM_ERR_ARG, "batch * num_modes * roi^2 must stay below 2^31 (batch %d, %d modes, roi %d)", d->batch,
+                d->num_modes, d->roi);
   if (d->num_gpts < 1) return fail(QEM_ERR_ARG, "num_gpts must be >= 1 (got %d)", d->num_gpts);
   if (d->segment_slices < 0) return fail(QEM_ERR_ARG, "segment_slices must be >= 0 (got %d)", d->segment_slices);
   if (d->loss_type < QEM_LOSS_L2_AMPLITUDE || d->loss_type > QEM_LOSS_POISSON)

@@ -70,6 +70,7 @@ def test_checkpoint_workspace_formula():
     ("roi", 96, nat.QEM_ERR_UNSUPPORTED), ("num_slices", 0, nat.QEM_ERR_ARG), ("batch", 0, nat.QEM_ERR_ARG),
     ("obj_rows", 64, nat.QEM_ERR_ARG), ("loss_type", 9, nat.QEM_ERR_ARG), ("mean_intensity", 0.0, nat.QEM_ERR_ARG),
     ("num_gpts", 0, nat.QEM_ERR_ARG), ("num_modes", 0, nat.QEM_ERR_ARG),
+    ("batch", 32768, nat.QEM_ERR_ARG),  # 32768 x 4 modes x 128^2 = 2^31 far-field elements
 ])
 def test_descriptor_validation(field, value, code):
     lib = nat.load()
