@@ -13,6 +13,7 @@ reference's default learning rates (object 5e-3, probe 1e-3; ``object_models.py:
 
 from __future__ import annotations
 
+import math
 import time
 from math import ceil
 from typing import Callable, Optional
@@ -109,9 +110,22 @@ DATASET_CONSTRAINTS = {
 }
 
 
-# the object and probe soft-constraint weights and their defaults (object_models.py:274-287,
-# probe_models.py:431-436)
+# the object's constraint switches and their defaults (ObjectConstraints.DEFAULT_CONSTRAINTS,
+# object_models.py:274-287): the hard part shapes the object every batch, the soft weights add losses
+OBJECT_HARD_CONSTRAINTS = {
+    "positivity": True,
+    "fix_potential_baseline": False,
+    "fix_potential_baseline_factor": 1.0,
+    "identical_slices": False,
+    "apply_fov_mask": False,
+    "gaussian_sigma": None,  # pixels
+    "butterworth_order": 4,
+    "q_lowpass": None,  # 1/A
+    "q_highpass": None,  # 1/A
+}
 OBJECT_SOFT_CONSTRAINTS = {"tv_weight_z": 0.0, "tv_weight_xy": 0.0, "surface_zero_weight": 0.0}
+OBJECT_CONSTRAINTS = {**OBJECT_HARD_CONSTRAINTS, **OBJECT_SOFT_CONSTRAINTS}
+# the probe's soft-constraint weight (probe_models.py:431-436)
 PROBE_SOFT_CONSTRAINTS = {"tv_weight": 0.0}
 
 
@@ -167,17 +181,99 @@ def probe_soft_loss(probe: torch.Tensor, tv_weight: float = 0.0) -> torch.Tensor
     return tv_weight * tv
 
 
-def object_hard_constraints(obj: torch.Tensor, obj_type: str = "complex") -> torch.Tensor:
-    """ObjectConstraints.apply_hard_constraints for the default constraint set (object_models.py:289-339),
-    in torch (the loop uses :func:`quantem_amd.constraints.object_constraints` on the GPU)."""
+def gaussian_blur_2d(t: torch.Tensor, sigma: float) -> torch.Tensor:
+    """Separable Gaussian blur of every (H, W) slice of an (S, H, W) tensor, real and imaginary parts
+    alike: a normalised kernel of 2 ceil(3 sigma) + 1 taps, zero padding (object_models.py:437-479)."""
+    k = int(2 * math.ceil(3 * sigma) + 1)
+    if k % 2 == 0:
+        k += 1
+    ax = torch.arange(-k // 2 + 1.0, k // 2 + 1.0, device=t.device)
+    g = torch.exp(-0.5 * (ax / sigma) ** 2)
+    g = g / g.sum()
+    kh, kv = g.view(1, 1, -1, 1), g.view(1, 1, 1, -1)
+
+    def blur(x):
+        x = torch.nn.functional.conv2d(x.unsqueeze(1), kh, padding=(k // 2, 0))
+        return torch.nn.functional.conv2d(x, kv, padding=(0, k // 2)).squeeze(1)
+
+    return torch.complex(blur(t.real), blur(t.imag)) if t.is_complex() else blur(t)
+
+
+def butterworth_filter(t: torch.Tensor, sampling, q_lowpass=None, q_highpass=None, order: int = 4,
+                       real_out: bool = False) -> torch.Tensor:
+    """Butterworth low / high pass of every slice about its mean (object_models.py:481-521): envelope
+    1 / (1 + (q/q_low)^(2 order)) times 1 - 1 / (1 + (q/q_high)^(2 order)) on |q| (1/A, fftfreq of the
+    slice shape at ``sampling`` A/px); the real part is kept for a potential."""
+    qx = torch.fft.fftfreq(t.shape[-2], sampling[0], device=t.device)
+    qy = torch.fft.fftfreq(t.shape[-1], sampling[1], device=t.device)
+    qya, qxa = torch.meshgrid(qy, qx, indexing="xy")
+    qra = torch.sqrt(qxa ** 2 + qya ** 2)
+    env = torch.ones_like(qra)
+    if q_highpass:
+        env = env * (1 - 1 / (1 + (qra / q_highpass) ** (2 * order)))
+    if q_lowpass:
+        env = env * (1 / (1 + (qra / q_lowpass) ** (2 * order)))
+    mean = t.mean(dim=(-2, -1), keepdim=True)
+    out = torch.fft.ifft2(torch.fft.fft2(t - mean) * env) + mean
+    return out.real if real_out else out
+
+
+def object_hard_constraints(obj: torch.Tensor, obj_type: str = "complex", constraints: Optional[dict] = None,
+                            mask: Optional[torch.Tensor] = None, sampling=None) -> torch.Tensor:
+    """ObjectConstraints.apply_hard_constraints (object_models.py:289-339) in torch, returning the object in
+    its own type (a potential stays real):
+
+    * "complex" / "pure_phase": amplitude clamp(|O|, 0, 1) (or 1), phase minus its mean; with
+      ``apply_fov_mask`` and a mask, amp * mask * exp(i phase * mask);
+    * "potential": minus a baseline (the minimum, or the mean below half the mask's maximum, times
+      ``fix_potential_baseline_factor``, without gradient) when ``fix_potential_baseline``, then
+      clamped at zero when ``positivity``;
+    * then the FOV mask, a Gaussian blur (``gaussian_sigma`` px), a Butterworth filter (``q_lowpass`` /
+      ``q_highpass``, needs ``sampling``) and, for ``identical_slices``, every slice set to the mean
+      slice without gradient, in that order.
+
+    The default constraint set of the complex types is also a HIP kernel pair
+    (:func:`quantem_amd.constraints.object_constraints`), which the loop uses on the GPU.
+    """
+    c = dict(OBJECT_HARD_CONSTRAINTS)
+    if constraints:
+        c.update({k: v for k, v in constraints.items() if k in OBJECT_HARD_CONSTRAINTS})
+    use_mask = mask is not None and c["apply_fov_mask"]
     if obj_type in ("complex", "pure_phase"):
         amp = torch.clamp(torch.abs(obj), 0.0, 1.0) if obj_type == "complex" else 1.0
         ang = obj.angle()
         phase = ang - ang.mean()
-        return amp * torch.exp(1.0j * phase)
-    if obj_type == "potential":
-        return torch.exp(1.0j * torch.clamp(obj, min=0.0))
-    raise ValueError(f"unknown obj_type {obj_type}")
+        out = amp * mask * torch.exp(1.0j * phase * mask) if use_mask else amp * torch.exp(1.0j * phase)
+    elif obj_type == "potential":
+        offset = 0.0
+        if c["fix_potential_baseline"]:
+            if mask is not None:
+                background = mask < 0.5 * mask.max()
+                offset = obj[..., background].mean() if background.any() else obj.min()
+            else:
+                offset = obj.min()
+            offset = offset.detach() * c["fix_potential_baseline_factor"]
+        out = torch.clamp(obj - offset, min=0.0) if c["positivity"] else obj - offset
+    else:
+        raise ValueError(f"unknown obj_type {obj_type}")
+    return _object_filters(out, obj_type, c, mask, sampling)
+
+
+def _object_filters(out: torch.Tensor, obj_type: str, c: dict, mask, sampling) -> torch.Tensor:
+    """The steps of the hard constraints after the amplitude / phase or baseline step."""
+    if c["apply_fov_mask"] and mask is not None:
+        out = out * mask
+    if c.get("gaussian_sigma") is not None:
+        out = gaussian_blur_2d(out, c["gaussian_sigma"])
+    if c["q_lowpass"] or c["q_highpass"]:
+        if sampling is None:
+            raise ValueError("q_lowpass / q_highpass need the object sampling (A/px)")
+        out = butterworth_filter(out, sampling, c["q_lowpass"], c["q_highpass"], c["butterworth_order"],
+                                 real_out=obj_type == "potential")
+    if out.shape[0] > 1 and c["identical_slices"]:
+        with torch.no_grad():
+            out[:] = torch.mean(out, dim=0, keepdim=True)
+    return out
 
 
 def orthogonalize_probe(probe: torch.Tensor) -> torch.Tensor:
@@ -217,7 +313,8 @@ class Reconstruction:
 
     def __init__(self, positions, targets, propagators, probe, obj_shape, mean_intensity: float,
                  mask=None, obj_type: str = "complex", orthogonalize: bool = True, upsample_factor: float = 1.0,
-                 obj_init: Optional[torch.Tensor] = None, descan=None, device="cuda"):
+                 obj_init: Optional[torch.Tensor] = None, descan=None, object_constraints: Optional[dict] = None,
+                 fov_mask=None, sampling=None, device="cuda"):
         dev = torch.device(device)
         self.device = dev
         self.positions = torch.as_tensor(positions, dtype=torch.float32, device=dev).clone().contiguous()
@@ -236,6 +333,11 @@ class Reconstruction:
                              f"{tuple(self.descan.shape)}")
         self.obj_shape = tuple(int(x) for x in obj_shape)
         self.obj_type = obj_type
+        # the object's constraints (hard and soft, the reference's keys), its FOV mask (H, W) and sampling
+        self.object_constraints = dict(OBJECT_CONSTRAINTS)
+        self.set_object_constraints(object_constraints)
+        self.fov_mask = None if fov_mask is None else torch.as_tensor(fov_mask, dtype=torch.float32, device=dev)
+        self.sampling = None if sampling is None else tuple(float(x) for x in sampling)
         self.orthogonalize = orthogonalize
         self.upsample_factor = float(upsample_factor)
         # a potential object is real and starts at zero; the others are complex and start flat at 1
@@ -262,11 +364,30 @@ class Reconstruction:
     def probe(self) -> torch.Tensor:
         return torch.view_as_complex(self._probe)
 
+    def set_object_constraints(self, constraints: Optional[dict]) -> None:
+        """Updates the object's constraint switches / weights (keys of :data:`OBJECT_CONSTRAINTS`)."""
+        if constraints:
+            unknown = set(constraints) - set(OBJECT_CONSTRAINTS)
+            if unknown:
+                raise KeyError(f"unknown object constraints {sorted(unknown)}; known: {sorted(OBJECT_CONSTRAINTS)}")
+            self.object_constraints.update(constraints)
+
+    @property
+    def obj_hard(self) -> torch.Tensor:
+        """The object after its hard constraints, in its own type (``obj_model.obj`` of the reference)."""
+        c = self.object_constraints
+        if (self.device.type == "cuda" and self.obj_type in ("complex", "pure_phase")
+                and not (c["apply_fov_mask"] and self.fov_mask is not None)):
+            # the amplitude / phase step as the HIP kernel pair, the optional filters after it in torch
+            return _object_filters(object_constraints(self.obj, self.obj_type), self.obj_type, c, self.fov_mask,
+                                   self.sampling)
+        return object_hard_constraints(self.obj, self.obj_type, c, self.fov_mask, self.sampling)
+
     @property
     def obj_constrained(self) -> torch.Tensor:
-        if self.device.type == "cuda" and self.obj_type in ("complex", "pure_phase"):
-            return object_constraints(self.obj, self.obj_type)  # the HIP kernel pair
-        return object_hard_constraints(self.obj, self.obj_type)
+        """The transmission the forward model takes: the constrained object, exp(i V) for a potential."""
+        o = self.obj_hard
+        return torch.exp(1.0j * o) if self.obj_type == "potential" else o
 
     @property
     def probe_constrained(self) -> torch.Tensor:
@@ -319,10 +440,11 @@ class Reconstruction:
         stepped once per epoch with the epoch's mean loss (``ptychography.py:278-279``).  The learning
         rate of every model after each epoch goes to ``self.lr_history``.
 
-        ``object_constraints`` / ``probe_constraints`` override the soft-constraint weights
-        (:data:`OBJECT_SOFT_CONSTRAINTS`, :data:`PROBE_SOFT_CONSTRAINTS`); their losses join every
-        batch loss (``ptychography.py:231-232``), so with ``autograd=False`` they are reported but do not
-        act, as in the reference.
+        ``object_constraints`` updates the object's constraints (hard switches and soft weights,
+        :data:`OBJECT_CONSTRAINTS`; they persist on the instance), ``probe_constraints`` the probe's soft
+        weight (:data:`PROBE_SOFT_CONSTRAINTS`).  The soft losses join every batch loss
+        (``ptychography.py:231-232``), so with ``autograd=False`` they are reported but do not act, as in
+        the reference.
         """
         def merged(defaults, given, what):
             out = dict(defaults)
@@ -334,9 +456,10 @@ class Reconstruction:
             return out
 
         cons = merged(DATASET_CONSTRAINTS, dataset_constraints, "dataset")
-        ocons = merged(OBJECT_SOFT_CONSTRAINTS, object_constraints, "object")
+        self.set_object_constraints(object_constraints)
+        ocons = self.object_constraints
         pcons = merged(PROBE_SOFT_CONSTRAINTS, probe_constraints, "probe")
-        obj_soft = any(v != 0 for v in ocons.values())
+        obj_soft = any(ocons[k] != 0 for k in OBJECT_SOFT_CONSTRAINTS)
         learn_dset = dataset_lr is not None and (learn_positions or learn_descan)
         if learn_dset and not autograd:
             raise ValueError("learning positions or descan needs autograd=True (the analytical update gives "
@@ -368,12 +491,11 @@ class Reconstruction:
         batcher = SimpleBatcher(self.num_gpts, batch_size, rng=seed, val_ratio=val_ratio, val_mode=val_mode)
         tv_weight = float(cons["descan_tv_weight"]) if use_descan else 0.0
 
-        def soft_loss(oc, pc):
+        def soft_loss(oh, pc):
             """the batch's soft-constraint loss (object, probe, descan), or None when every weight is 0"""
             terms = []
             if obj_soft:
-                o = torch.clamp(self.obj, min=0.0) if self.obj_type == "potential" else oc
-                terms.append(object_soft_loss(o, self.obj_type, ocons["tv_weight_z"], ocons["tv_weight_xy"],
+                terms.append(object_soft_loss(oh, self.obj_type, ocons["tv_weight_z"], ocons["tv_weight_xy"],
                                               ocons["surface_zero_weight"]))
             if pcons["tv_weight"] != 0:
                 terms.append(probe_soft_loss(pc, pcons["tv_weight"]))
@@ -400,22 +522,24 @@ class Reconstruction:
                         o.zero_grad(set_to_none=True)
                     it, pos, desc = batch_inputs(idx)
                     if autograd:
-                        oc, pc = self.obj_constrained, self.probe_constrained
+                        oh, pc = self.obj_hard, self.probe_constrained
+                        oc = torch.exp(1.0j * oh) if self.obj_type == "potential" else oh
                         loss = multislice_loss(oc, pc, pos, self.propagators, self.targets[it], self.mask, desc,
                                                num_gpts=self.num_gpts, mean_intensity=self.mean_intensity,
                                                loss_type=loss_type, segment_slices=segment_slices)
-                        soft = soft_loss(oc, pc)
+                        soft = soft_loss(oh, pc)
                         if soft is not None:
                             loss = loss + soft
                         loss.backward()
                     else:
                         with torch.no_grad():
-                            oc, pc = self.obj_constrained, self.probe_constrained
+                            oh, pc = self.obj_hard, self.probe_constrained
+                            oc = torch.exp(1.0j * oh) if self.obj_type == "potential" else oh
                             loss, og, pg = analytical_gradients(
                                 oc, pc, pos, self.propagators, self.targets[it], self.mask, None,
                                 num_gpts=self.num_gpts, mean_intensity=self.mean_intensity, loss_type=loss_type,
                                 obj_type=self.obj_type)
-                            soft = soft_loss(oc, pc)
+                            soft = soft_loss(oh, pc)
                             if soft is not None:
                                 loss = loss + soft
                         self._obj.grad = _real_view(og).contiguous()

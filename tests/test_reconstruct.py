@@ -313,3 +313,71 @@ def test_object_tv_constraint_smooths_the_reconstruction(cuda):
     print(f"phase TV without / with the constraint: {tv0:.4g} / {tv1:.4g}", flush=True)
     assert np.isfinite(h1).all()
     assert tv1 < tv0
+
+
+def test_gaussian_blur_matches_scipy():
+    """gaussian_blur_2d = per-slice Gaussian filter with zero padding and ceil(3 sigma) taps each side."""
+    from scipy import ndimage
+
+    rng = np.random.default_rng(4)
+    x = rng.normal(size=(2, 21, 17))
+    for sigma in (0.7, 1.5, 2.0):
+        r = int(np.ceil(3 * sigma))
+        ref = ndimage.gaussian_filter(x, sigma=(0, sigma, sigma), mode="constant", cval=0.0, truncate=r / sigma)
+        got = rc.gaussian_blur_2d(torch.from_numpy(x).float(), sigma).numpy()
+        assert np.abs(got - ref).max() < 1e-5
+    z = (x + 1j * rng.normal(size=x.shape)).astype(np.complex64)
+    gz = rc.gaussian_blur_2d(torch.from_numpy(z), 1.0).numpy()
+    refz = (ndimage.gaussian_filter(z.real, (0, 1, 1), mode="constant", truncate=3.0) +
+            1j * ndimage.gaussian_filter(z.imag, (0, 1, 1), mode="constant", truncate=3.0))
+    assert np.abs(gz - refz).max() < 1e-5
+
+
+def test_butterworth_low_plus_high_is_identity_and_halves_at_cutoff():
+    rng = np.random.default_rng(5)
+    x = torch.from_numpy(rng.normal(size=(2, 32, 24))).float() + 3.0
+    s = (0.5, 0.25)
+    lp = rc.butterworth_filter(x, s, q_lowpass=0.3)
+    hp = rc.butterworth_filter(x, s, q_highpass=0.3)
+    mean = x.mean(dim=(-2, -1), keepdim=True)
+    assert torch.allclose((lp + hp - mean).real, x, atol=1e-5)  # the two envelopes sum to one
+    assert torch.allclose(lp.mean(dim=(-2, -1)).real, mean[..., 0, 0], atol=1e-5)  # the mean passes
+    # a cosine exactly at the cutoff along rows: 4 periods over 32 rows of 0.5 A -> q = 0.25 1/A
+    r = torch.arange(32, dtype=torch.float32).view(32, 1).expand(32, 24)
+    cosine = torch.cos(2 * np.pi * 4 * r / 32).unsqueeze(0)
+    for order in (1, 4):
+        out = rc.butterworth_filter(cosine, s, q_lowpass=0.25, order=order, real_out=True)
+        assert torch.allclose(out, 0.5 * cosine, atol=1e-5)
+
+
+def test_object_hard_constraint_options():
+    rng = np.random.default_rng(6)
+    v = torch.from_numpy(rng.normal(size=(3, 8, 8))).float() + 0.5
+    out = rc.object_hard_constraints(v, "potential", {"fix_potential_baseline": True})
+    assert float(out.min()) == 0.0 and torch.allclose(out, torch.clamp(v - v.min(), min=0.0))
+    assert torch.equal(rc.object_hard_constraints(v, "potential", {"positivity": False}), v)
+    same = rc.object_hard_constraints(v, "potential", {"identical_slices": True})
+    assert torch.allclose(same[0], same[2]) and torch.allclose(same[1], torch.clamp(v, min=0.0).mean(0))
+    o = torch.exp(1j * torch.from_numpy(rng.normal(size=(1, 8, 8))).float()).to(torch.complex64) * 0.8
+    mask = torch.zeros(8, 8)
+    mask[2:6, 2:6] = 1.0
+    om = rc.object_hard_constraints(o, "complex", {"apply_fov_mask": True}, mask=mask)
+    plain = rc.object_hard_constraints(o, "complex")
+    assert torch.all(om[:, mask == 0] == 0)
+    assert torch.allclose(om[:, mask == 1], plain[:, mask == 1], atol=1e-6)
+    with pytest.raises(ValueError):
+        rc.object_hard_constraints(o, "complex", {"q_lowpass": 0.5})  # no sampling
+
+
+@pytest.mark.gpu
+def test_gpu_constraints_with_filters_match_torch(cuda):
+    """The loop's object: HIP amplitude / phase step + torch filters equals the torch restatement."""
+    rng = np.random.default_rng(7)
+    o0 = (rng.uniform(0.5, 1.2, (2, 64, 64)) * np.exp(1j * rng.normal(0, 0.3, (2, 64, 64)))).astype(np.complex64)
+    cons = {"gaussian_sigma": 1.0, "q_lowpass": 0.8, "identical_slices": True}
+    rec = rc.Reconstruction(np.zeros((1, 2), np.float32), np.zeros((1, 32, 32), np.float32), np.ones((1, 32, 32),
+                            np.complex64), np.ones((1, 32, 32), np.complex64), (2, 64, 64), 1.0, obj_init=o0,
+                            object_constraints=cons, sampling=(0.4, 0.4), device=cuda)
+    got = rec.obj_hard.detach().cpu()
+    ref = rc.object_hard_constraints(torch.from_numpy(o0), "complex", cons, sampling=(0.4, 0.4))
+    assert torch.allclose(got, ref, atol=2e-5)
