@@ -291,6 +291,24 @@ def orthogonalize_probe(probe: torch.Tensor) -> torch.Tensor:
     return torch.complex(out.real[order], out.imag[order])
 
 
+def center_probe(probe: torch.Tensor) -> torch.Tensor:
+    """ProbeConstraints' centre-of-mass constraint (probe_models.py:469-481): each corner-centred mode is
+    Fourier-shifted (ptycho_utils.py:128-176, fftfreq ramps) by minus the offset of the centre of mass of
+    its fftshifted intensity from (N/2, N/2)."""
+    nr, nc = probe.shape[-2:]
+    inten = torch.fft.fftshift(probe.abs().square(), dim=(-2, -1))
+    rows = torch.arange(nr, device=probe.device, dtype=inten.dtype)
+    cols = torch.arange(nc, device=probe.device, dtype=inten.dtype)
+    tot = inten.sum(dim=(-2, -1))
+    d_r = (inten * rows[:, None]).sum(dim=(-2, -1)) / tot - nr // 2
+    d_c = (inten * cols[None, :]).sum(dim=(-2, -1)) / tot - nc // 2
+    kr = torch.fft.fftfreq(nr, d=1.0, device=probe.device).to(inten.dtype)
+    kc = torch.fft.fftfreq(nc, d=1.0, device=probe.device).to(inten.dtype)
+    ramp = (torch.exp(2.0j * math.pi * kr[None, :, None] * d_r[:, None, None]) *
+            torch.exp(2.0j * math.pi * kc[None, None, :] * d_c[:, None, None]))
+    return torch.fft.ifft2(torch.fft.fft2(probe) * ramp.to(probe.dtype))
+
+
 def projected_phase(obj: torch.Tensor) -> torch.Tensor:
     """Sum over slices of the transmission phase."""
     return torch.sum(torch.angle(obj), dim=0)
@@ -314,7 +332,7 @@ class Reconstruction:
     def __init__(self, positions, targets, propagators, probe, obj_shape, mean_intensity: float,
                  mask=None, obj_type: str = "complex", orthogonalize: bool = True, upsample_factor: float = 1.0,
                  obj_init: Optional[torch.Tensor] = None, descan=None, object_constraints: Optional[dict] = None,
-                 fov_mask=None, sampling=None, device="cuda"):
+                 fov_mask=None, sampling=None, center_probe: bool = False, device="cuda"):
         dev = torch.device(device)
         self.device = dev
         self.positions = torch.as_tensor(positions, dtype=torch.float32, device=dev).clone().contiguous()
@@ -339,6 +357,7 @@ class Reconstruction:
         self.fov_mask = None if fov_mask is None else torch.as_tensor(fov_mask, dtype=torch.float32, device=dev)
         self.sampling = None if sampling is None else tuple(float(x) for x in sampling)
         self.orthogonalize = orthogonalize
+        self.center_probe = center_probe
         self.upsample_factor = float(upsample_factor)
         # a potential object is real and starts at zero; the others are complex and start flat at 1
         odt = torch.float32 if obj_type == "potential" else torch.complex64
@@ -391,7 +410,10 @@ class Reconstruction:
 
     @property
     def probe_constrained(self) -> torch.Tensor:
-        return orthogonalize_probe(self.probe) if (self.orthogonalize and self.probe.shape[0] > 1) else self.probe
+        """ProbeConstraints.apply_hard_constraints (probe_models.py:452-459): Gram-Schmidt (default on),
+        then centring (default off)."""
+        p = orthogonalize_probe(self.probe) if (self.orthogonalize and self.probe.shape[0] > 1) else self.probe
+        return center_probe(p) if self.center_probe else p
 
     def apply_dataset_constraints(self, constraints: dict) -> None:
         """The dataset's hard constraints on the scan positions (``dataset_models.py:597-613``), in place:

@@ -381,3 +381,25 @@ def test_gpu_constraints_with_filters_match_torch(cuda):
     got = rec.obj_hard.detach().cpu()
     ref = rc.object_hard_constraints(torch.from_numpy(o0), "complex", cons, sampling=(0.4, 0.4))
     assert torch.allclose(got, ref, atol=2e-5)
+
+
+def test_center_probe_moves_the_intensity_centroid_to_the_middle():
+    n = 32
+    f = np.fft.fftfreq(n, d=1.0 / n)
+    rr, cc = np.meshgrid(f, f, indexing="ij")
+    modes = []
+    for dr, dc in ((3.25, -2.0), (-1.5, 4.75)):
+        modes.append(np.exp(-((rr - dr) ** 2 + (cc - dc) ** 2) / (2 * 2.0 ** 2)) * np.exp(0.3j * rr))
+    p = torch.from_numpy(np.stack(modes).astype(np.complex64))
+    out = rc.center_probe(p)
+    inten = np.fft.fftshift(np.abs(out.numpy()) ** 2, axes=(-2, -1))
+    idx = np.arange(n)
+    for m in range(2):
+        tot = inten[m].sum()
+        assert abs((inten[m] * idx[:, None]).sum() / tot - n // 2) < 1e-3
+        assert abs((inten[m] * idx[None, :]).sum() / tot - n // 2) < 1e-3
+    # a Fourier shift is unitary: every mode keeps its norm
+    assert torch.allclose(out.abs().square().sum(dim=(-2, -1)), p.abs().square().sum(dim=(-2, -1)), rtol=1e-5)
+    # an already centred mode stays put
+    p0 = torch.from_numpy(np.exp(-(rr ** 2 + cc ** 2) / 8.0).astype(np.complex64))[None]
+    assert torch.allclose(rc.center_probe(p0), p0, atol=1e-5)
