@@ -332,8 +332,9 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
   // One slice back: object-gradient patch conj(psi_s) g_x (written over psi_s), then
   // g_psi = conj(O_s) g_x, in chunks of CH elements whose psi and object loads are issued before
   // their stores.  At N = 128 the next slice's psi values (all 32; 20 in the descan variant, whose
-  // extra state would otherwise spill) are loaded before this slice's propagation transforms, so that
-  // the next step's HBM latency hides behind the transforms; smaller N keep their occupancy instead.
+  // extra state would otherwise spill) are loaded during this slice's propagation, after the
+  // propagator multiply (issued before it, they would hold up the propagator loads: one vmcnt), so
+  // that their HBM latency hides behind the inverse transform; smaller N keep their occupancy instead.
   constexpr int CH = PTS < 16 ? PTS : 16;
   constexpr int PF = N >= 128 ? (DESCAN ? 20 : PTS) : 0;
   cf pre[PF > 0 ? PF : 1];
@@ -362,16 +363,16 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
   for (int s = p.s1 - 1; s >= p.s0; --s) {
     slice_step(s, s < p.s1 - 1);
     if (s > 0) {
-      if (PF > 0 && s - 1 >= p.s0) {  // the next slice down belongs to this launch's segment
-        const cf* pn = p.psi + ((size_t)item * p.wslots + (s - 1 - p.s0)) * N * N;
-#pragma unroll
-        for (int i = 0; i < PF; ++i) pre[i] = ld_stream(pn + f.p1_off(i));
-      }
       // g_x <- ifft2(fft2(g_psi) conj(prop_{s-1})), the adjoint of the forward propagation
       f.fwd(v, lds);
       const cf* __restrict__ pr = p.prop + (size_t)(s - 1) * N * N;
 #pragma unroll
       for (int e = 0; e < PTS; ++e) v[e] = cscale(cmulc(v[e], pr[f.p3_off(e)]), inv_n2);
+      if (PF > 0 && s - 1 >= p.s0) {  // the next slice down belongs to this launch's segment
+        const cf* pn = p.psi + ((size_t)item * p.wslots + (s - 1 - p.s0)) * N * N;
+#pragma unroll
+        for (int i = 0; i < PF; ++i) pre[i] = ld_stream(pn + f.p1_off(i));
+      }
       f.inv(v, lds);
     }
   }
