@@ -240,6 +240,25 @@ def test_analytical_update_matches_oracle(cuda, n, s, m, b):
     assert rel(pg.cpu().numpy(), ref_p) < GRAD_TOL
 
 
+@pytest.mark.parametrize("n", [64, 256])
+def test_analytical_update_potential_matches_oracle(cuda, n):
+    """The analytical update of a potential object (O = exp(iV)): real(-i conj(O) update)."""
+    print(f"analytical potential n={n}", flush=True)
+    case = _case_with_targets(N=n, S=2, M=2, B=3, seed=400 + n, descan=False)
+    v = np.random.default_rng(n).uniform(0.0, 0.5, case["obj"].shape)
+    o = np.exp(1j * v).astype(np.complex64)
+    dev = cuda
+    loss, og, pg = qp.analytical_gradients(_t(o, dev), _t(case["probe"], dev), _t(case["positions"], dev),
+                                           _t(case["propagators"], dev), _t(case["targets"], dev),
+                                           _t(case["mask"], dev), num_gpts=case["num_gpts"],
+                                           mean_intensity=case["mean_intensity"], obj_type="potential")
+    ref_o, ref_p = po.analytical_update(o, case["probe"], case["positions"], case["propagators"], case["targets"],
+                                        obj_type="potential")
+    assert og.dtype == torch.float32
+    assert rel(og.cpu().numpy(), ref_o.real) < GRAD_TOL
+    assert rel(pg.cpu().numpy(), ref_p) < GRAD_TOL
+
+
 @pytest.mark.parametrize("n,s,m,b,h,w", [(32, 2, 2, 70, 56, 48), (64, 2, 2, 130, 96, 80), (32, 2, 2, 5, 40, 6400)])
 def test_object_reduce_chunks_and_wide_rows(cuda, n, s, m, b, h, w):
     """Batches larger than the row reduce's N-pattern chunks, and rows wider than its LDS row

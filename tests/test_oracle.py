@@ -248,3 +248,21 @@ def test_analytical_update_relates_to_autograd(M):
     energy = np.sum(np.abs(fw["patches"]) ** 2, axis=(-2, -1)).max()
     ortho = 2 * 16 if M == 1 else 1 / (2 * 16)
     assert rel(pg * (2 * scale * energy * ortho), ag["probe"]) < 1e-4
+
+
+def test_analytical_potential_branch_is_the_phase_chain_rule():
+    """For a potential V with O = exp(iV) the reference's update term is real(-i conj(O) x) of the complex
+    term (object_models.py:701-715); the normalisation is shared, so pixel by pixel the potential update
+    equals real(-i conj(O) * complex update), and the probe update does not change."""
+    case = make_case(N=16, S=2, M=2, B=4, seed=12)
+    v = np.random.default_rng(12).uniform(0.0, 0.6, case["obj"].shape)
+    o = np.exp(1j * v).astype(np.complex64)
+    I = po.forward(o, case["probe"], case["positions"], case["propagators"])["I"]
+    case = attach_targets(case, I * 1.3)
+    og_c, pg_c = po.analytical_update(o, case["probe"], case["positions"], case["propagators"], case["targets"])
+    og_v, pg_v = po.analytical_update(o, case["probe"], case["positions"], case["propagators"], case["targets"],
+                                      obj_type="potential")
+    o128 = o.astype(np.complex128)
+    assert np.abs(og_v.imag).max() == 0.0
+    assert rel(og_v.real, np.real(-1j * np.conj(o128) * og_c)) < 1e-12
+    assert rel(pg_v, pg_c) < 1e-14

@@ -403,3 +403,18 @@ def test_center_probe_moves_the_intensity_centroid_to_the_middle():
     # an already centred mode stays put
     p0 = torch.from_numpy(np.exp(-(rr ** 2 + cc ** 2) / 8.0).astype(np.complex64))[None]
     assert torch.allclose(rc.center_probe(p0), p0, atol=1e-5)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("autograd", [True, False])
+def test_potential_object_reconstruction(cuda, autograd):
+    """A real potential object (transmission exp(iV), positivity) reconstructs in both update modes."""
+    spec, prob, amps, mean_I = _small_problem(cuda)
+    rec = rc.Reconstruction(prob["positions"], amps, None, prob["probe"], (1,) + spec.obj_shape, mean_I,
+                            obj_type="potential", upsample_factor=spec.scan_step, device=cuda)
+    hist = rec.reconstruct(6, 32, obj_lr=2e-2, learn_probe=False, seed=0, autograd=autograd,
+                           object_constraints={"tv_weight_xy": 1e-3})
+    assert np.isfinite(hist).all() and hist[-1] < 0.9 * hist[0]
+    assert rec.obj.dtype == torch.float32 and float(rec.obj_hard.detach().min()) >= 0.0
+    o = rec.obj_constrained
+    assert o.dtype == torch.complex64 and torch.allclose(o.abs(), torch.ones_like(o.abs()), atol=1e-5)
