@@ -74,6 +74,7 @@ def test_scheduler_factory_matches_reference_defaults():
     assert isinstance(lin, torch.optim.lr_scheduler.LinearLR) and lin.total_iters == 10
     assert o.param_groups[0]["lr"] == pytest.approx(0.004)  # starts at 0.1 x
     for _ in range(10):
+        o.step()  # no gradients: a no-op, in the order torch expects
         step_scheduler(lin, 1.0)
     assert o.param_groups[0]["lr"] == pytest.approx(0.04)
     with pytest.raises(ValueError):
@@ -86,5 +87,6 @@ def test_plateau_schedule_halves_after_patience():
     o = torch.optim.SGD([torch.nn.Parameter(torch.zeros(1))], lr=1.0)
     p = make_scheduler(o, {"type": "plateau", "patience": 2, "cooldown": 0})
     for loss in (1.0, 1.0, 1.0, 1.0):
+        o.step()
         step_scheduler(p, loss)
     assert o.param_groups[0]["lr"] == pytest.approx(0.5)  # 3 epochs without improvement after the best
