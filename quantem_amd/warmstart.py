@@ -410,6 +410,11 @@ def main(argv=None) -> int:
     t_train = time.perf_counter() - t0
     log(f"trained {args.epochs} epochs in {t_train:.1f} s")
     results = []
+    if args.eval > 0:
+        # one untimed evaluation of an object outside the held-out seeds first, so that one-time costs
+        # (first use of kernels and allocator growth) stay out of the timed runs
+        evaluate(net, norm, sim, 99999, epochs=args.rec_epochs)
+        torch.cuda.synchronize()
     for k in range(args.eval):
         r = evaluate(net, norm, sim, 100000 + k, epochs=args.rec_epochs)
         log(f"seed {100000 + k}: flat final NMSE {r['target_nmse']:.4f} in {r['t_flat']:.2f} s; warm start "
