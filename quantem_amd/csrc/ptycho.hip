@@ -342,24 +342,52 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
     const cf* O = p.obj + (size_t)s * HW;
     cf* ps = p.psi + ((size_t)item * p.wslots + (s - p.s0)) * N * N;
     oo.opaque();
+    if constexpr (PF == PTS && 2 * CH == PTS) {
+      // every psi value is in registers: the second chunk's object loads are issued before the first
+      // chunk's stores, so that waiting for them does not wait for those stores (one vmcnt counter)
+      cf o[CH];
 #pragma unroll
-    for (int e0 = 0; e0 < PTS; e0 += CH) {
-      cf t[CH], o[CH];
+      for (int i = 0; i < CH; ++i) o[i] = obj_at(O, oo.at(i));
 #pragma unroll
       for (int i = 0; i < CH; ++i) {
-        if (e0 + i < PF && have_pre)
-          t[i] = pre[e0 + i];
-        else
-          t[i] = ld_stream(ps + f.p1_off(e0 + i));
-        o[i] = obj_at(O, oo.at(e0 + i));
+        pre[i] = cconjmul(pre[i], v[i]);  // the patch value, in place of psi
+        v[i] = cconjmul(o[i], v[i]);
       }
 #pragma unroll
+      for (int i = 0; i < CH; ++i) o[i] = obj_at(O, oo.at(CH + i));
+#pragma unroll
+      for (int i = 0; i < CH; ++i) st_stream(ps + f.p1_off(i), pre[i]);
+#pragma unroll
       for (int i = 0; i < CH; ++i) {
-        st_stream(ps + f.p1_off(e0 + i), cconjmul(t[i], v[e0 + i]));
-        v[e0 + i] = cconjmul(o[i], v[e0 + i]);
+        st_stream(ps + f.p1_off(CH + i), cconjmul(pre[CH + i], v[CH + i]));
+        v[CH + i] = cconjmul(o[i], v[CH + i]);
+      }
+    } else {
+#pragma unroll
+      for (int e0 = 0; e0 < PTS; e0 += CH) {
+        cf t[CH], o[CH];
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
+          if (e0 + i < PF && have_pre)
+            t[i] = pre[e0 + i];
+          else
+            t[i] = ld_stream(ps + f.p1_off(e0 + i));
+          o[i] = obj_at(O, oo.at(e0 + i));
+        }
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
+          st_stream(ps + f.p1_off(e0 + i), cconjmul(t[i], v[e0 + i]));
+          v[e0 + i] = cconjmul(o[i], v[e0 + i]);
+        }
       }
     }
   };
+  if constexpr (PF == PTS && 2 * CH == PTS) {
+    // the top slice's psi values, loaded as every later slice's are prefetched
+    const cf* pt = p.psi + ((size_t)item * p.wslots + (p.s1 - 1 - p.s0)) * N * N;
+#pragma unroll
+    for (int i = 0; i < PF; ++i) pre[i] = ld_stream(pt + f.p1_off(i));
+  }
   for (int s = p.s1 - 1; s >= p.s0; --s) {
     slice_step(s, s < p.s1 - 1);
     if (s > 0) {
