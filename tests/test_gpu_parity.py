@@ -270,3 +270,37 @@ def test_object_reduce_chunks_and_wide_rows(cuda, n, s, m, b, h, w):
     assert rel(got["I"], ref["I"]) < FWD_TOL
     for k in ("obj", "probe", "positions"):
         assert rel(got[k], ref[k]) < GRAD_TOL, k
+
+
+def test_headline_size_batch_is_the_sum_of_its_parts(cuda):
+    """The headline shape (8 x 384^2, 128^2, 4 modes, batch 256): per-pattern intensities are
+    bitwise those of four 64-pattern sub-batches, and loss and gradients are the sub-batch sums
+    (the loss carries P / B, hence the factor B_sub / B).  The oracle pins two patterns' intensities."""
+    print("headline-size batch decomposition", flush=True)
+    case = make_case(N=128, S=8, M=4, B=256, H=384, W=384, seed=8)
+    dev = cuda
+    props = _t(case["propagators"], dev)
+    obj_t, probe_t = _t(case["obj"], dev), _t(case["probe"], dev)
+    pos_t = _t(case["positions"], dev)
+    tg = qp.forward_intensities(obj_t * 0.98, probe_t, pos_t, props).clamp_min(0).sqrt()
+    args = dict(num_gpts=4096, mean_intensity=float((tg ** 2).sum(dim=(-2, -1)).mean()))
+
+    def run(sl):
+        o = obj_t.clone().requires_grad_(True)
+        p = probe_t.clone().requires_grad_(True)
+        q = pos_t[sl].clone().requires_grad_(True)
+        loss, I = qp.multislice_loss(o, p, q, props, tg[sl], None, None, return_intensities=True, **args)
+        loss.backward()
+        return loss.detach(), I, o.grad, p.grad, q.grad
+
+    full = run(slice(0, 256))
+    parts = [run(slice(k, k + 64)) for k in range(0, 256, 64)]
+    f = 64 / 256
+    assert torch.equal(full[1], torch.cat([pt[1] for pt in parts]))
+    assert abs(float(full[0]) - f * sum(float(pt[0]) for pt in parts)) < 1e-5 * abs(float(full[0]))
+    for i in (2, 3):
+        ref = f * sum(pt[i] for pt in parts)
+        assert rel(full[i].cpu().numpy(), ref.cpu().numpy()) < 1e-5, i
+    assert rel(full[4].cpu().numpy(), (f * torch.cat([pt[4] for pt in parts])).cpu().numpy()) < 1e-5
+    sub = po.forward(case["obj"], case["probe"], case["positions"][:2], case["propagators"])["I"]
+    assert rel(full[1][:2].cpu().numpy(), sub) < FWD_TOL
