@@ -199,6 +199,18 @@ struct NoOp {
 
 // pk[m] = fft2(probe[m]).
 __global__ __launch_bounds__(T) void k_spectrum256(Params p) {
+  if ((int)blockIdx.x >= p.M) {
+    // the blocks past the modes fill the shift-ramp table: ramps[b][0][k] = exp(-2 pi i f_k dr_b),
+    // ramps[b][1][k] = exp(-2 pi i f_k dc_b), read by the forward's first and the adjoint's last pass
+    const int idx = ((int)blockIdx.x - p.M) * T + (int)threadIdx.x;
+    if (idx >= p.B * 2 * N) return;
+    const int b = idx / (2 * N), r = idx - b * 2 * N, k = r & (N - 1);
+    int r0, c0;
+    float dr, dc;
+    origin_and_shift<N>(p.pos, b, r0, c0, dr, dc);
+    p.ramps[idx] = expm2pi(fftfreq1<N>(k) * (r < N ? dr : dc));
+    return;
+  }
   __shared__ Smem sm;
   init_tw(sm);
   const int m = blockIdx.x;
@@ -233,11 +245,12 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(1, 1))) void 
 
   if (far) {
     // psi_0 = ifft2(fft2(P_m) * ramp_b): inverse column DFT here, inverse row DFT in the first row pass
+    const cf* rr = p.ramps + (size_t)b * 2 * N;  // row factors, then column factors
     col_pass(p.pk + (size_t)m * N2, Wb, sm, [&](int kc, cf(&v)[16]) {
-      const float fc = fftfreq1<N>(kc);
+      const cf rc = rr[N + kc];
       const int jj = col_j();
 #pragma unroll
-      for (int e = 0; e < 16; ++e) v[e] = cscale(cmul(v[e], ramp2(fftfreq1<N>(jj + 16 * e), fc, dr, dc)), inv_n2);
+      for (int e = 0; e < 16; ++e) v[e] = cscale(cmul(v[e], cmul(rr[jj + 16 * e], rc)), inv_n2);
       fft256<true, true>(v, sm);
     });
   }
@@ -424,16 +437,18 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(1, 1))) void 
   // G = fft2(g_psi_0); probe-gradient spectrum conj(ramp_b) G over X, position partials
   // (2 pi / N^2) sum f Im(conj(G) P_m ramp_b)
   const cf* pk = p.pk + (size_t)m * N2;
+  const cf* rr = p.ramps + (size_t)b * 2 * N;  // row factors, then column factors
   double ar = 0.0, ac = 0.0;
   col_pass(Wb, Xs, sm, [&](int kc, cf(&v)[16]) {
     fft256<false, true>(v, sm);
     const int jj = col_j();
     const float fc = fftfreq1<N>(kc);
+    const cf rc = rr[N + kc];
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
       const int kr = jj + 16 * e;
       const float fr = fftfreq1<N>(kr);
-      const cf R = ramp2(fr, fc, dr, dc);
+      const cf R = cmul(rr[kr], rc);
       const cf G = v[e];
       const float im = cconjmul(G, cmul(pk[kr * N + kc], R)).y;
       ar += (double)(fr * im);
@@ -516,7 +531,9 @@ __global__ __launch_bounds__(T) void k_center256(int P, const float* __restrict_
 }  // namespace n256
 
 int launch_chain256_forward(const Params& p, hipStream_t st) {
-  hipLaunchKernelGGL(n256::k_spectrum256, dim3(p.M), dim3(n256::T), 0, st, p);
+  // M probe-spectrum blocks, then the blocks of the (B, 2, N) shift-ramp table
+  const int ramp_blocks = (p.B * 2 * n256::N + n256::T - 1) / n256::T;
+  hipLaunchKernelGGL(n256::k_spectrum256, dim3(p.M + ramp_blocks), dim3(n256::T), 0, st, p);
   if (int rc = check_last_launch("spectrum256")) return rc;
   return launch_forward256_kernel(p, st);
 }
