@@ -314,6 +314,14 @@ struct Fft2 {
 
   // Forward 2-D DFT (unnormalised), P1 -> P3.
   __device__ __forceinline__ void fwd(cf (&v)[PTS], cf* lds) const {
+    fwd_groups(v, lds, [](auto) {}, [](auto) {});
+  }
+
+  // fwd() with hooks around each pass-3 register group h (registers h*RR3 .. h*RR3 + RR3 - 1, one
+  // frequency column, rows p3_row): pre(h) runs before the group's transform and post(h) after it, so
+  // that a pointwise step can issue its global loads for a group while that group is transformed.
+  template <typename Pre, typename Post>
+  __device__ __forceinline__ void fwd_groups(cf (&v)[PTS], cf* lds, Pre&& pre, Post&& post) const {
     // pass 1
     sfor<0, RC1>([&](auto c1) { Dft<RR1, false>::template run<RC1>(&v[c1.value]); });
     sfor<0, RC1>([&](auto c1) { v[RC1 + c1.value] = cmul(v[RC1 + c1.value], tw_rp); });
@@ -347,7 +355,11 @@ struct Fft2 {
       sfor<0, RR3>([&](auto r3) { v[h.value * RR3 + r3.value] = lds[e2p3[h.value] + B3 * r3.value]; });
     });
     // pass 3
-    sfor<0, H3>([&](auto h) { Dft<RR3, false>::template run<1>(&v[h.value * RR3]); });
+    sfor<0, H3>([&](auto h) {
+      pre(h);
+      Dft<RR3, false>::template run<1>(&v[h.value * RR3]);
+      post(h);
+    });
   }
 
   // Inverse 2-D DFT without the 1/N^2 factor (= N^2 * ifft2), P3 -> P1.

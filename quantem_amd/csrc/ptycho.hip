@@ -158,10 +158,23 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
 #pragma unroll
     for (int e = 0; e < PTS; ++e) v[e] = cmul(o[e], v[e]);
     if (s + 1 < p.s1) {
-      f.fwd(v, lds);
+      // propagation: the propagator values of each pass-3 register group are loaded before that
+      // group's last transform stage, so that their L2 latency hides behind it
       const cf* pr = p.prop + (size_t)s * N * N;
+      cf q[F::RR3];
+      f.fwd_groups(
+          v, lds,
+          [&](auto h) {
 #pragma unroll
-      for (int e = 0; e < PTS; ++e) v[e] = cscale(cmul(v[e], pr[f.p3_off(e)]), inv_n2);
+            for (int i = 0; i < F::RR3; ++i) q[i] = pr[f.p3_off(h.value * F::RR3 + i)];
+          },
+          [&](auto h) {
+#pragma unroll
+            for (int i = 0; i < F::RR3; ++i) {
+              const int e = h.value * F::RR3 + i;
+              v[e] = cscale(cmul(v[e], q[i]), inv_n2);
+            }
+          });
       f.inv(v, lds);
     }
   }
@@ -391,13 +404,29 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
   for (int s = p.s1 - 1; s >= p.s0; --s) {
     slice_step(s, s < p.s1 - 1);
     if (s > 0) {
-      // g_x <- ifft2(fft2(g_psi) conj(prop_{s-1})), the adjoint of the forward propagation
-      f.fwd(v, lds);
+      // g_x <- ifft2(fft2(g_psi) conj(prop_{s-1})), the adjoint of the forward propagation, with the
+      // propagator loads of each pass-3 register group issued before that group's last stage
       const cf* __restrict__ pr = p.prop + (size_t)(s - 1) * N * N;
+      cf q[F::RR3];
+      f.fwd_groups(
+          v, lds,
+          [&](auto h) {
 #pragma unroll
-      for (int e = 0; e < PTS; ++e) v[e] = cscale(cmulc(v[e], pr[f.p3_off(e)]), inv_n2);
-      if (PF > 0 && s - 1 >= p.s0) {  // the next slice down belongs to this launch's segment
-        const cf* pn = p.psi + ((size_t)item * p.wslots + (s - 1 - p.s0)) * N * N;
+            for (int i = 0; i < F::RR3; ++i) q[i] = pr[f.p3_off(h.value * F::RR3 + i)];
+          },
+          [&](auto h) {
+#pragma unroll
+            for (int i = 0; i < F::RR3; ++i) {
+              const int e = h.value * F::RR3 + i;
+              v[e] = cscale(cmulc(v[e], q[i]), inv_n2);
+            }
+          });
+      if constexpr (PF > 0) {
+        // Branch-free: at a segment bottom (s - 1 < s0) this reads the segment's first slot and the
+        // values go unused.  Under a condition the compiler moved the transform's arithmetic below
+        // the prefetch, behind a register reload that waits for every load in flight.
+        const int sl = s - 1 >= p.s0 ? s - 1 - p.s0 : 0;
+        const cf* pn = p.psi + ((size_t)item * p.wslots + sl) * N * N;
 #pragma unroll
         for (int i = 0; i < PF; ++i) pre[i] = ld_stream(pn + f.p1_off(i));
       }

@@ -409,11 +409,16 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(1, 1))) void 
     if (s > 0) {
       // g_x <- ifft2(fft2(g_psi) conj(prop_{s-1})); at a segment bottom it goes to X for the segment below
       const cf* pr = p.prop + (size_t)(s - 1) * N2;
+      // (the propagator column is loaded before the column DFT, which hides its latency: measured
+      // faster here, slower in the forward's column pass)
       col_pass<true>(Wb, s == p.s0 ? Xs : Wb, sm, [&](int kc, cf(&v)[16]) {
-        fft256<false, true>(v, sm);
         const int jj = col_j();
+        cf q[16];
 #pragma unroll
-        for (int e = 0; e < 16; ++e) v[e] = cscale(cmulc(v[e], pr[(jj + 16 * e) * N + kc]), inv_n2);
+        for (int e = 0; e < 16; ++e) q[e] = pr[(jj + 16 * e) * N + kc];
+        fft256<false, true>(v, sm);
+#pragma unroll
+        for (int e = 0; e < 16; ++e) v[e] = cscale(cmulc(v[e], q[e]), inv_n2);
         fft256<true, true>(v, sm);
       });
     }
