@@ -93,7 +93,10 @@ struct ObjOffsets {
 // ---------------------------------------------------------------------------------------------
 // Forward multislice for one (pattern, mode): psi_0 = ifft2(fft2(P_m) * ramp_b);
 // for s: x_s = O_s[patch] * psi_s; psi_{s+1} = ifft2(fft2(x_s) * prop_s); X = fft2_ortho(x_{S-1}).
-template <int N, bool DESCAN>
+// ALL: every incident wave is kept (FWD_STORE_ALL, FWD_RECOMPUTE), so the stores are not under a
+// branch; behind one, the wait-count analysis merged the two paths and made the transmission wait
+// for the stores ahead of it.
+template <int N, bool DESCAN, bool ALL>
 __global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
   __shared__ cf lds[Fft2<N>::LDS_TOTAL];
   using F = Fft2<N>;
@@ -113,6 +116,10 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
     const cf* src = p.ckpt + ((size_t)item * p.nck + p.s0 / p.ck) * N * N;
 #pragma unroll
     for (int e = 0; e < PTS; ++e) v[e] = src[f.p1_off(e)];
+    // through an empty asm, so that the values enter the slice loop as plain registers: as loads
+    // still possibly in flight at the loop head they made every wave store of every slice wait
+#pragma unroll
+    for (int e = 0; e < PTS; ++e) asm volatile("" : "+v"(v[e].x), "+v"(v[e].y));
   } else {
     // shifted probe: spectrum * ramp / N^2, inverse transform (row and column ramps from the table)
     const cf* pk = p.pk + (size_t)m * N * N;
@@ -136,7 +143,7 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
   for (int s = p.s0; s < p.s1; ++s) {
     const cf* O = p.obj + (size_t)s * HW;
     cf* dst = nullptr;
-    if (mode == FWD_STORE_ALL || mode == FWD_RECOMPUTE)
+    if (ALL)  // FWD_STORE_ALL, FWD_RECOMPUTE
       dst = p.psi + ((size_t)item * p.wslots + (s - p.s0)) * N * N;
     else if (mode == FWD_CHECKPOINT && s % p.ck == 0)
       dst = p.ckpt + ((size_t)item * p.nck + s / p.ck) * N * N;
@@ -150,7 +157,7 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
 #pragma unroll
       for (int e = 0; e < PTS; ++e) o[e] = obj_at(O, oo.at(e));
     }
-    if (dst != nullptr) {
+    if (ALL || dst != nullptr) {
 #pragma unroll
       for (int e = 0; e < PTS; ++e) st_stream(dst + f.p1_off(e), v[e]);
     }
@@ -1008,10 +1015,16 @@ int launch_forward(const qem::Params& p, const Layout& L, hipStream_t st) {
   hipLaunchKernelGGL(qem::k_probe_spectrum<N>, dim3(p.M + ramp_blocks), dim3(F::T), 0, st, p);
   if (int rc = check_launch("probe_spectrum")) return rc;
   prof_mark(false, 0, st);
-  if (p.descan != nullptr)
-    hipLaunchKernelGGL((qem::k_forward<N, true>), dim3(p.B * p.M), dim3(F::T), 0, st, p);
+  // every incident wave kept: the store-every-slice instantiation (no branch around the stores)
+  const bool all = p.fwd_mode == qem::FWD_STORE_ALL;
+  if (p.descan != nullptr && all)
+    hipLaunchKernelGGL((qem::k_forward<N, true, true>), dim3(p.B * p.M), dim3(F::T), 0, st, p);
+  else if (p.descan != nullptr)
+    hipLaunchKernelGGL((qem::k_forward<N, true, false>), dim3(p.B * p.M), dim3(F::T), 0, st, p);
+  else if (all)
+    hipLaunchKernelGGL((qem::k_forward<N, false, true>), dim3(p.B * p.M), dim3(F::T), 0, st, p);
   else
-    hipLaunchKernelGGL((qem::k_forward<N, false>), dim3(p.B * p.M), dim3(F::T), 0, st, p);
+    hipLaunchKernelGGL((qem::k_forward<N, false, false>), dim3(p.B * p.M), dim3(F::T), 0, st, p);
   prof_mark(false, 1, st);
   if (int rc = check_launch("forward")) return rc;
   const int nparts = (L.nloss + qem::kLossTiles - 1) / qem::kLossTiles;
@@ -1117,7 +1130,7 @@ int launch_adjoint(const qem::Params& p, hipStream_t st) {
   prof_mark(true, 0, st);
   int rc = for_each_segment(p, [&](const qem::Params& q) {
     if (q.fwd_mode == qem::FWD_RECOMPUTE) {
-      hipLaunchKernelGGL((qem::k_forward<N, false>), dim3(q.B * q.M), dim3(F::T), 0, st, q);
+      hipLaunchKernelGGL((qem::k_forward<N, false, true>), dim3(q.B * q.M), dim3(F::T), 0, st, q);
       if (int r = check_launch("forward (recompute)")) return r;
     }
     if (q.descan != nullptr)
