@@ -364,6 +364,14 @@ struct Fft2 {
 
   // Inverse 2-D DFT without the 1/N^2 factor (= N^2 * ifft2), P3 -> P1.
   __device__ __forceinline__ void inv(cf (&v)[PTS], cf* lds) const {
+    inv_hooked(v, lds, [] {});
+  }
+
+  // inv() with a hook run once the last exchange has been read, before the final register pass, so
+  // that a caller can issue global loads (for the elementwise step after the transform) whose latency
+  // that pass then hides.
+  template <typename Pre1>
+  __device__ __forceinline__ void inv_hooked(cf (&v)[PTS], cf* lds, Pre1&& pre1) const {
     sfor<0, H3>([&](auto h) { Dft<RR3, true>::template run<1>(&v[h.value * RR3]); });
     sfor<0, H3>([&](auto h) {
       sfor<0, RR3>([&](auto r3) { lds[e2p3[h.value] + B3 * r3.value] = v[h.value * RR3 + r3.value]; });
@@ -389,6 +397,7 @@ struct Fft2 {
     sfor<0, RR1>([&](auto k1) {
       sfor<0, RC1>([&](auto kc0) { v[k1.value * RC1 + kc0.value] = lds[e1p1 + A3 * k1.value + A1 * kc0.value]; });
     });
+    pre1();
     sfor<1, RC1>([&](auto kc0) {
       sfor<0, RR1>([&](auto k1) { v[k1.value * RC1 + kc0.value] = cmulc(v[k1.value * RC1 + kc0.value], twl[twc_base + kc0.value]); });
     });

@@ -140,6 +140,14 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
   const bool far = mode != FWD_RECOMPUTE;
   const size_t HW = (size_t)p.H * p.W;
   const float inv_n2 = 1.0f / (float)(N * N);
+  // At N = 128 the object values of slice s + 1 are loaded inside the inverse transform of slice s's
+  // propagation (after its last exchange, before its final register pass), which hides their L2
+  // latency; at smaller N other resident waves hide it and the loads stay at the slice start.
+  constexpr int OK = N >= 128 ? PTS : 0;  // object values loaded ahead
+  cf o[PTS];
+  oo.opaque();
+#pragma unroll
+  for (int e = 0; e < OK; ++e) o[e] = obj_at(p.obj + (size_t)p.s0 * HW, oo.at(e));
   for (int s = p.s0; s < p.s1; ++s) {
     const cf* O = p.obj + (size_t)s * HW;
     cf* dst = nullptr;
@@ -152,10 +160,9 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
     // through the transforms.  The object loads are issued before the wave stores, because loads and
     // stores share one vmcnt counter and a load issued after a store waits for that store as well.
     oo.opaque();
-    cf o[PTS];
     if (!last_only) {
 #pragma unroll
-      for (int e = 0; e < PTS; ++e) o[e] = obj_at(O, oo.at(e));
+      for (int e = OK; e < PTS; ++e) o[e] = obj_at(O, oo.at(e));
     }
     if (ALL || dst != nullptr) {
 #pragma unroll
@@ -182,7 +189,12 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
               v[e] = cscale(cmul(v[e], q[i]), inv_n2);
             }
           });
-      f.inv(v, lds);
+      const cf* On = O + HW;
+      f.inv_hooked(v, lds, [&] {
+        oo.opaque();
+#pragma unroll
+        for (int e = 0; e < OK; ++e) o[e] = obj_at(On, oo.at(e));
+      });
     }
   }
   if (!far) return;
