@@ -367,32 +367,22 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
   // extra state would otherwise spill) are loaded during this slice's propagation, after the
   // propagator multiply (issued before it, they would hold up the propagator loads: one vmcnt), so
   // that their HBM latency hides behind the inverse transform; smaller N keep their occupancy instead.
+  // With all of them prefetched (N = 128 without descan) the slice's object values are loaded ahead
+  // too, inside that inverse transform (after its last exchange), and the slice step only stores.
   constexpr int CH = PTS < 16 ? PTS : 16;
   constexpr int PF = N >= 128 ? (DESCAN ? 20 : PTS) : 0;
+  constexpr bool AHEAD = PF == PTS;
   cf pre[PF > 0 ? PF : 1];
+  cf ob[AHEAD ? PTS : 1];
   auto slice_step = [&](int s, bool have_pre) {
     const cf* O = p.obj + (size_t)s * HW;
     cf* ps = p.psi + ((size_t)item * p.wslots + (s - p.s0)) * N * N;
     oo.opaque();
-    if constexpr (PF == PTS && 2 * CH == PTS) {
-      // every psi value is in registers: the second chunk's object loads are issued before the first
-      // chunk's stores, so that waiting for them does not wait for those stores (one vmcnt counter)
-      cf o[CH];
+    if constexpr (AHEAD) {
 #pragma unroll
-      for (int i = 0; i < CH; ++i) o[i] = obj_at(O, oo.at(i));
-#pragma unroll
-      for (int i = 0; i < CH; ++i) {
-        pre[i] = cconjmul(pre[i], v[i]);  // the patch value, in place of psi
-        v[i] = cconjmul(o[i], v[i]);
-      }
-#pragma unroll
-      for (int i = 0; i < CH; ++i) o[i] = obj_at(O, oo.at(CH + i));
-#pragma unroll
-      for (int i = 0; i < CH; ++i) st_stream(ps + f.p1_off(i), pre[i]);
-#pragma unroll
-      for (int i = 0; i < CH; ++i) {
-        st_stream(ps + f.p1_off(CH + i), cconjmul(pre[CH + i], v[CH + i]));
-        v[CH + i] = cconjmul(o[i], v[CH + i]);
+      for (int i = 0; i < PTS; ++i) {
+        st_stream(ps + f.p1_off(i), cconjmul(pre[i], v[i]));
+        v[i] = cconjmul(ob[i], v[i]);
       }
     } else {
 #pragma unroll
@@ -414,11 +404,14 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
       }
     }
   };
-  if constexpr (PF == PTS && 2 * CH == PTS) {
-    // the top slice's psi values, loaded as every later slice's are prefetched
+  if constexpr (AHEAD) {
+    // the top slice's psi and object values, loaded as every later slice's are prefetched
     const cf* pt = p.psi + ((size_t)item * p.wslots + (p.s1 - 1 - p.s0)) * N * N;
 #pragma unroll
     for (int i = 0; i < PF; ++i) pre[i] = ld_stream(pt + f.p1_off(i));
+    oo.opaque();
+#pragma unroll
+    for (int i = 0; i < PTS; ++i) ob[i] = obj_at(p.obj + (size_t)(p.s1 - 1) * HW, oo.at(i));
   }
   for (int s = p.s1 - 1; s >= p.s0; --s) {
     slice_step(s, s < p.s1 - 1);
@@ -449,7 +442,14 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
 #pragma unroll
         for (int i = 0; i < PF; ++i) pre[i] = ld_stream(pn + f.p1_off(i));
       }
-      f.inv(v, lds);
+      const cf* On = p.obj + (size_t)(s - 1) * HW;
+      f.inv_hooked(v, lds, [&] {
+        if constexpr (AHEAD) {
+          oo.opaque();
+#pragma unroll
+          for (int i = 0; i < PTS; ++i) ob[i] = obj_at(On, oo.at(i));
+        }
+      });
     }
   }
   if (p.s0 > 0) {
