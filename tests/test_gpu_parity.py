@@ -102,7 +102,7 @@ def test_loss_types(cuda, loss_type):
         assert rel(got[k], ref[k]) < GRAD_TOL, k
 
 
-@pytest.mark.parametrize("n", [64, 256])
+@pytest.mark.parametrize("n", [64, 128, 256])
 def test_no_grad_forward_matches(cuda, n):
     case = _case_with_targets(N=n, S=3, M=2, B=3, seed=3)
     with torch.no_grad():
