@@ -87,19 +87,27 @@ __device__ __forceinline__ void fft256(cf (&v)[16], Smem& sm) {
   Dft<16, INV>::template run<1>(v);                                     // over jj -> k2
 }
 
+struct NoPre {
+  __device__ __forceinline__ void operator()(int) const {}
+};
+
 // Row pass over the 256 rows of src (CH rows per chunk): v <- row (columns j + 16 e); inverse row DFT
 // if PRE_INV; op(r, v) on the natural-order row; forward row DFT if POST_FWD; row -> dst (if not null).
 // EARLY_NEXT: each chunk issues the next chunk's loads before its own stores (a load issued after a
 // store waits for it: one vmcnt counter) and carries them in registers.  Measured faster for both
 // passes of the adjoint and for the forward's row pass; with it in the forward's column pass as well
-// the forward's registers spill into AGPRs and it runs slower.
-template <bool PRE_INV, bool POST_FWD, bool EARLY_NEXT = false, typename Op>
-__device__ __forceinline__ void row_pass(const cf* src, cf* dst, Smem& sm, Op&& op) {
+// the forward's registers spill into AGPRs and it runs slower.  pre(r) (EARLY_NEXT only) runs for the
+// first chunk's rows before the loop and for each next chunk's rows right after that chunk's loads,
+// so that op can use values it loaded one chunk ahead (their latency then hides behind the next
+// chunk's inverse row DFT).
+template <bool PRE_INV, bool POST_FWD, bool EARLY_NEXT = false, typename Op, typename Pre = NoPre>
+__device__ __forceinline__ void row_pass(const cf* src, cf* dst, Smem& sm, Op&& op, Pre&& pre = Pre()) {
   const int j = row_j();
   if constexpr (EARLY_NEXT) {
     cf v[16];
 #pragma unroll
     for (int e = 0; e < 16; ++e) v[e] = src[row_vec() * N + j + 16 * e];
+    pre(row_vec());
     for (int r0 = 0; r0 < N; r0 += CH) {
       const int r = r0 + row_vec();
       if constexpr (PRE_INV) fft256<true, false>(v, sm);
@@ -109,6 +117,7 @@ __device__ __forceinline__ void row_pass(const cf* src, cf* dst, Smem& sm, Op&& 
       if (r0 + CH < N) {  // the chunks touch disjoint rows
 #pragma unroll
         for (int e = 0; e < 16; ++e) nxt[e] = src[(r + CH) * N + j + 16 * e];
+        pre(r + CH);
       }
       if (dst != nullptr) {
 #pragma unroll
@@ -260,6 +269,28 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(1, 1))) void 
     cf* keep = slot(s);
     const bool transmit = far || s + 1 < p.s1;  // a recomputed segment needs only its incident waves
     const bool exit_ramp = DESCAN && far && s == p.S - 1;
+    // the transmit pass (op_t) gets its object values one chunk ahead (pre_obj); the other row passes
+    // (a recomputed segment's first and last slice) use op, which loads them in the chunk itself
+    cf ov[16];
+    auto pre_obj = [&](int r) {
+      const unsigned orow = obj_row_off(r0, r, p.H, p.W);
+#pragma unroll
+      for (int e = 0; e < 16; ++e) ov[e] = obj_at(O, orow + obj_col_off(q, e, p.W));
+    };
+    auto op_t = [&](int r, cf(&v)[16]) {
+      if (keep != nullptr) {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) st_stream(keep + r * N + j + 16 * e, v[e]);
+      }
+#pragma unroll
+      for (int e = 0; e < 16; ++e) v[e] = cmul(ov[e], v[e]);
+      if (exit_ramp) {
+        const float fr = fftfreq1<N>(r);
+#pragma unroll
+        for (int e = 0; e < 16; ++e)
+          v[e] = cmul(v[e], ramp2(fr, fftfreq1<N>(j + 16 * e), p.descan[2 * b], p.descan[2 * b + 1]));
+      }
+    };
     auto op = [&](int r, cf(&v)[16]) {
       // object values first: a load issued after the wave stores would wait for them (one vmcnt)
       cf o[16];
@@ -291,7 +322,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(1, 1))) void 
       else
         row_pass<false, false>(ck, nullptr, sm, op);
     } else if (transmit) {
-      row_pass<true, true, true>(Wb, Wb, sm, op);
+      row_pass<true, true, true>(Wb, Wb, sm, op_t, pre_obj);
     } else {
       row_pass<true, false>(Wb, nullptr, sm, op);
     }
@@ -391,21 +422,26 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(1, 1))) void 
     const cf* O = p.obj + (size_t)s * HW;
     cf* A = p.psi + ((size_t)item * p.wslots + (s - p.s0)) * N2;
     // g_x of slice s in natural order: object-gradient patch conj(psi_s) g over psi_s, g_psi = conj(O_s) g
-    row_pass<true, true, true>((!top && s == p.s1 - 1) ? (const cf*)Xs : (const cf*)Wb, Wb, sm, [&](int r, cf(&v)[16]) {
-      const unsigned orow = obj_row_off(r0, r, p.H, p.W);
-      // every load of the row chunk before its stores (a load issued after a store waits for it)
-      cf ps[16], o[16];
+    // the stored-wave values of a row chunk are loaded one chunk ahead (pre); the object values
+    // stay in the chunk itself: loading both ahead pushed the kernel into AGPR copies (slower)
+    cf pv[16];
+    row_pass<true, true, true>(
+        (!top && s == p.s1 - 1) ? (const cf*)Xs : (const cf*)Wb, Wb, sm,
+        [&](int r, cf(&v)[16]) {
+          const unsigned orow = obj_row_off(r0, r, p.H, p.W);
+          cf o[16];
 #pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        ps[e] = ld_stream(A + r * N + j + 16 * e);
-        o[e] = obj_at(O, orow + obj_col_off(q, e, p.W));
-      }
+          for (int e = 0; e < 16; ++e) o[e] = obj_at(O, orow + obj_col_off(q, e, p.W));
 #pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        st_stream(A + r * N + j + 16 * e, cconjmul(ps[e], v[e]));
-        v[e] = cconjmul(o[e], v[e]);
-      }
-    });
+          for (int e = 0; e < 16; ++e) {
+            st_stream(A + r * N + j + 16 * e, cconjmul(pv[e], v[e]));
+            v[e] = cconjmul(o[e], v[e]);
+          }
+        },
+        [&](int r) {
+#pragma unroll
+          for (int e = 0; e < 16; ++e) pv[e] = ld_stream(A + r * N + j + 16 * e);
+        });
     if (s > 0) {
       // g_x <- ifft2(fft2(g_psi) conj(prop_{s-1})); at a segment bottom it goes to X for the segment below
       const cf* pr = p.prop + (size_t)(s - 1) * N2;
