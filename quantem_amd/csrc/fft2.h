@@ -22,6 +22,10 @@
 // the same thread at the same addresses (and inv() ends / fwd() starts likewise in P1), so
 // fwd/inv alternation needs no extra barrier.  Two fwd() (or two inv()) in a row need a
 // __syncthreads() between them.
+//
+// Before each exchange, every register group's LDS writes are issued right after that group's last
+// stage (a scheduling barrier keeps them there), so the LDS drains them while the next group is
+// computed instead of after the whole pass.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <type_traits>
@@ -325,12 +329,11 @@ struct Fft2 {
     // pass 1
     sfor<0, RC1>([&](auto c1) { Dft<RR1, false>::template run<RC1>(&v[c1.value]); });
     sfor<0, RC1>([&](auto c1) { v[RC1 + c1.value] = cmul(v[RC1 + c1.value], tw_rp); });
-    sfor<0, RR1>([&](auto k1) { Dft<RC1, false>::template run<1>(&v[k1.value * RC1]); });
-    sfor<1, RC1>([&](auto kc0) {
-      sfor<0, RR1>([&](auto k1) { v[k1.value * RC1 + kc0.value] = cmul(v[k1.value * RC1 + kc0.value], twl[twc_base + kc0.value]); });
-    });
     sfor<0, RR1>([&](auto k1) {
+      Dft<RC1, false>::template run<1>(&v[k1.value * RC1]);
+      sfor<1, RC1>([&](auto kc0) { v[k1.value * RC1 + kc0.value] = cmul(v[k1.value * RC1 + kc0.value], twl[twc_base + kc0.value]); });
       sfor<0, RC1>([&](auto kc0) { lds[e1p1 + A3 * k1.value + A1 * kc0.value] = v[k1.value * RC1 + kc0.value]; });
+      __builtin_amdgcn_sched_barrier(0);  // keep the group's exchange writes here
     });
     lds_barrier();
     sfor<0, RR2>([&](auto r2) {
@@ -341,14 +344,11 @@ struct Fft2 {
     lds_barrier();
     // pass 2
     sfor<0, RR2>([&](auto r2) { Dft<RC2, false>::template run<1>(&v[r2.value * RC2]); });
-    sfor<0, RC2>([&](auto kc1) { Dft<RR2, false>::template run<RC2>(&v[kc1.value]); });
-    sfor<1, RR2>([&](auto k2) {
-      sfor<0, RC2>([&](auto kc1) { v[k2.value * RC2 + kc1.value] = cmul(v[k2.value * RC2 + kc1.value], twl[twr_base + k2.value]); });
-    });
-    sfor<0, RR2>([&](auto k2) {
-      sfor<0, RC2>([&](auto kc1) {
-        lds[e2p2 + B2 * k2.value + RC1 * kc1.value] = v[k2.value * RC2 + kc1.value];
-      });
+    sfor<0, RC2>([&](auto kc1) {
+      Dft<RR2, false>::template run<RC2>(&v[kc1.value]);
+      sfor<1, RR2>([&](auto k2) { v[k2.value * RC2 + kc1.value] = cmul(v[k2.value * RC2 + kc1.value], twl[twr_base + k2.value]); });
+      sfor<0, RR2>([&](auto k2) { lds[e2p2 + B2 * k2.value + RC1 * kc1.value] = v[k2.value * RC2 + kc1.value]; });
+      __builtin_amdgcn_sched_barrier(0);  // keep the group's exchange writes here
     });
     lds_barrier();
     sfor<0, H3>([&](auto h) {
@@ -372,9 +372,10 @@ struct Fft2 {
   // that pass then hides.
   template <typename Pre1>
   __device__ __forceinline__ void inv_hooked(cf (&v)[PTS], cf* lds, Pre1&& pre1) const {
-    sfor<0, H3>([&](auto h) { Dft<RR3, true>::template run<1>(&v[h.value * RR3]); });
     sfor<0, H3>([&](auto h) {
+      Dft<RR3, true>::template run<1>(&v[h.value * RR3]);
       sfor<0, RR3>([&](auto r3) { lds[e2p3[h.value] + B3 * r3.value] = v[h.value * RR3 + r3.value]; });
+      __builtin_amdgcn_sched_barrier(0);  // keep the group's exchange writes here
     });
     lds_barrier();
     sfor<0, RR2>([&](auto k2) {
@@ -387,11 +388,10 @@ struct Fft2 {
       sfor<0, RC2>([&](auto kc1) { v[k2.value * RC2 + kc1.value] = cmulc(v[k2.value * RC2 + kc1.value], twl[twr_base + k2.value]); });
     });
     sfor<0, RC2>([&](auto kc1) { Dft<RR2, true>::template run<RC2>(&v[kc1.value]); });
-    sfor<0, RR2>([&](auto r2) { Dft<RC2, true>::template run<1>(&v[r2.value * RC2]); });
     sfor<0, RR2>([&](auto r2) {
-      sfor<0, RC2>([&](auto c0i) {
-        lds[e1p2 + A2 * RR3 * r2.value + c0i.value] = v[r2.value * RC2 + c0i.value];
-      });
+      Dft<RC2, true>::template run<1>(&v[r2.value * RC2]);
+      sfor<0, RC2>([&](auto c0i) { lds[e1p2 + A2 * RR3 * r2.value + c0i.value] = v[r2.value * RC2 + c0i.value]; });
+      __builtin_amdgcn_sched_barrier(0);  // keep the group's exchange writes here
     });
     lds_barrier();
     sfor<0, RR1>([&](auto k1) {
