@@ -341,9 +341,9 @@ struct Fft2 {
         v[r2.value * RC2 + c0i.value] = lds[e1p2 + A2 * RR3 * r2.value + c0i.value];
       });
     });
-    lds_barrier();
     // pass 2
     sfor<0, RR2>([&](auto r2) { Dft<RC2, false>::template run<1>(&v[r2.value * RC2]); });
+    lds_barrier();  // every thread's exchange-1 reads are done before any exchange-2 write
     sfor<0, RC2>([&](auto kc1) {
       Dft<RR2, false>::template run<RC2>(&v[kc1.value]);
       sfor<1, RR2>([&](auto k2) { v[k2.value * RC2 + kc1.value] = cmul(v[k2.value * RC2 + kc1.value], twl[twr_base + k2.value]); });
@@ -383,11 +383,11 @@ struct Fft2 {
         v[k2.value * RC2 + kc1.value] = lds[e2p2 + B2 * k2.value + RC1 * kc1.value];
       });
     });
-    lds_barrier();
     sfor<1, RR2>([&](auto k2) {
       sfor<0, RC2>([&](auto kc1) { v[k2.value * RC2 + kc1.value] = cmulc(v[k2.value * RC2 + kc1.value], twl[twr_base + k2.value]); });
     });
     sfor<0, RC2>([&](auto kc1) { Dft<RR2, true>::template run<RC2>(&v[kc1.value]); });
+    lds_barrier();  // every thread's exchange-2 reads are done before any exchange-1 write
     sfor<0, RR2>([&](auto r2) {
       Dft<RC2, true>::template run<1>(&v[r2.value * RC2]);
       sfor<0, RC2>([&](auto c0i) { lds[e1p2 + A2 * RR3 * r2.value + c0i.value] = v[r2.value * RC2 + c0i.value]; });
