@@ -458,11 +458,11 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
     for (int e = 0; e < PTS; ++e) Xs[f.p1_off(e)] = v[e];
     return;
   }
-  f.fwd(v, lds);
   // G = fft2(g_psi_0) in natural order.  Over X goes the probe-gradient spectrum conj(ramp_b) G
   // (G itself for the analytical update: the reference sums g at the probe plane without the
   // shift's adjoint); the position-gradient partials are (2 pi / N^2) sum f Im(conj(G) P_m ramp_b).
   if (p.flags & QEM_FLAG_ANALYTIC) {
+    f.fwd(v, lds);
 #pragma unroll
     for (int e = 0; e < PTS; ++e) Xs[f.p3_off(e)] = v[e];
     return;
@@ -472,15 +472,31 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
   const cf rc = rr[N + f.kc];
   const float fc = fftfreq1<N>(f.kc);
   double ar = 0.0, ac = 0.0;
+  // per pass-3 register group: the probe-spectrum and ramp values are loaded before the group's last
+  // transform stage and used after it (terms summed in register order, as one loop would)
+  cf qp[F::RR3], qr[F::RR3];
+  f.fwd_groups(
+      v, lds,
+      [&](auto h) {
 #pragma unroll
-  for (int e = 0; e < PTS; ++e) {
-    const int kr = f.p3_row(e), off = f.p3_off(e);
-    const cf ramp = cmul(rr[kr], rc);
-    Xs[off] = cmulc(v[e], ramp);
-    const float im = cconjmul(v[e], cmul(pk[off], ramp)).y;
-    ar += (double)(fftfreq1<N>(kr) * im);
-    ac += (double)(fc * im);
-  }
+        for (int i = 0; i < F::RR3; ++i) {
+          const int e = h.value * F::RR3 + i;
+          qp[i] = pk[f.p3_off(e)];
+          qr[i] = rr[f.p3_row(e)];
+        }
+      },
+      [&](auto h) {
+#pragma unroll
+        for (int i = 0; i < F::RR3; ++i) {
+          const int e = h.value * F::RR3 + i;
+          const int kr = f.p3_row(e), off = f.p3_off(e);
+          const cf ramp = cmul(qr[i], rc);
+          Xs[off] = cmulc(v[e], ramp);
+          const float im = cconjmul(v[e], cmul(qp[i], ramp)).y;
+          ar += (double)(fftfreq1<N>(kr) * im);
+          ac += (double)(fc * im);
+        }
+      });
   if (p.flags & QEM_FLAG_GRAD_POSITIONS) {
     const double tr = block_sum<F::T>(ar, red);
     const double tc = block_sum<F::T>(ac, red);
