@@ -36,9 +36,9 @@ FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the adjoint kernel at this workload, from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE runs of this script (profiles/multislice_8x384_128px_4modes_hbm_traffic.md): FETCH_SIZE
-# 717473 KiB x 2.0 + WRITE_SIZE 1332627 KiB x 1.0, the factors being what the same counters read for
+# 722546 KiB x 2.0 + WRITE_SIZE 1324088 KiB x 1.0, the factors being what the same counters read for
 # torch.neg over 1 GiB (FETCH_SIZE 524311 KiB, WRITE_SIZE 1048576 KiB).
-ADJOINT_TRAFFIC_BYTES = (717473 * 2.0 + 1332627 * 1.0) * 1024
+ADJOINT_TRAFFIC_BYTES = (722546 * 2.0 + 1324088 * 1.0) * 1024
 
 
 def log(msg: str) -> None:
