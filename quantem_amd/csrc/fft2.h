@@ -344,11 +344,16 @@ struct Fft2 {
     // pass 2
     sfor<0, RR2>([&](auto r2) { Dft<RC2, false>::template run<1>(&v[r2.value * RC2]); });
     lds_barrier();  // every thread's exchange-1 reads are done before any exchange-2 write
-    sfor<0, RC2>([&](auto kc1) {
-      Dft<RR2, false>::template run<RC2>(&v[kc1.value]);
-      sfor<1, RR2>([&](auto k2) { v[k2.value * RC2 + kc1.value] = cmul(v[k2.value * RC2 + kc1.value], twl[twr_base + k2.value]); });
-      sfor<0, RR2>([&](auto k2) { lds[e2p2 + B2 * k2.value + RC1 * kc1.value] = v[k2.value * RC2 + kc1.value]; });
-      __builtin_amdgcn_sched_barrier(0);  // keep the group's exchange writes here
+    // two column groups at a time: their writes at the same k2 are RC1 slots apart and pair up
+    sfor<0, RC2 / 2>([&](auto g) {
+      sfor<2 * g.value, 2 * g.value + 2>([&](auto kc1) {
+        Dft<RR2, false>::template run<RC2>(&v[kc1.value]);
+        sfor<1, RR2>([&](auto k2) { v[k2.value * RC2 + kc1.value] = cmul(v[k2.value * RC2 + kc1.value], twl[twr_base + k2.value]); });
+      });
+      sfor<0, RR2>([&](auto k2) {
+        sfor<2 * g.value, 2 * g.value + 2>([&](auto kc1) { lds[e2p2 + B2 * k2.value + RC1 * kc1.value] = v[k2.value * RC2 + kc1.value]; });
+      });
+      __builtin_amdgcn_sched_barrier(0);  // keep the groups' exchange writes here
     });
     lds_barrier();
     sfor<0, H3>([&](auto h) {
