@@ -165,8 +165,9 @@ def main() -> int:
     P = data["num_gpts"]
     obj = torch.nn.Parameter(torch.ones((S, H, W), dtype=torch.complex64, device=dev))
     probe = torch.nn.Parameter(data["probe"].clone())
-    upsample = ((np.asarray(spec.scan) - 1) * spec.scan_step / (np.asarray(spec.scan) - 1)).mean()
-    scale = float(upsample**2 / 2)
+    # the reference's object-gradient rescale upsample_factor**2 / 2 (ptychography.py:330-332,
+    # dataset_models.py:451-456, 877-879): 252 / 64 = 3.9375 at this workload
+    scale = synth.spec_upsample_factor(spec) ** 2 / 2
     opt = qoptim.Adam([{"params": [obj], "lr": 5e-3, "grad_scale": scale}, {"params": [probe], "lr": 1e-3}])
     state = {"obj": obj, "probe": probe, "opt": opt}
     idx_list = [i.to(dev) for i in batches(P, B, args.seed)]

@@ -101,8 +101,10 @@ int qem_ptycho_forward_loss(const qem_ptycho_desc* desc, const void* obj, const 
 /* Gradient of the batch loss (torch convention: grad = dL/dRe + i dL/dIm), scaled by *dloss
  * (device float, the upstream gradient).  Replaces torch autograd of the chain above
  * (invoked at ptychography.py:328).  Must follow qem_ptycho_forward_loss with the same
- * descriptor (QEM_FLAG_STORE_WAVES set), inputs and workspace.
- *   grad_obj       (S, Ho, Wo) complex64, fully written (zero where no patch lands)
+ * descriptor (QEM_FLAG_STORE_WAVES set), inputs and workspace.  The call consumes the workspace
+ * (gradient patches overwrite the stored waves, probe-gradient spectra the far field): a second
+ * adjoint needs a new qem_ptycho_forward_loss first.
+ *   grad_obj      (S, Ho, Wo) complex64, fully written (zero where no patch lands)
  *   grad_probe     (M, N, N)   complex64
  *   grad_positions (B, 2) float32 or NULL (needs QEM_FLAG_GRAD_POSITIONS)
  *   grad_descan    (B, 2) float32 or NULL (needs QEM_FLAG_GRAD_DESCAN and descan != NULL) */

@@ -10,6 +10,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 REPO_DIR = os.path.dirname(PKG_DIR)
@@ -37,13 +38,11 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found (expected /opt/rocm/bin/hipcc)")
 
 
-def compile_command(out: str = LIB_PATH) -> list[str]:
+def _flags() -> list[str]:
     return [
-        hipcc(),
         f"--offload-arch={ARCH}",
         "-O3",
         "-std=c++17",
-        "-shared",
         "-fPIC",
         # packed FP32 (v_pk_*_f32) is disabled for all device code of this library
         "-Xclang",
@@ -52,22 +51,40 @@ def compile_command(out: str = LIB_PATH) -> list[str]:
         "-packed-fp32-ops",
         "-I",
         os.path.join(REPO_DIR, "include"),
-        "-o",
-        out,
-        *SOURCES,
     ]
 
 
+def compile_command(out: str = LIB_PATH) -> list[str]:
+    """One hipcc line that compiles and links every source (the serial form of :func:`build`)."""
+    return [hipcc(), *_flags(), "-shared", "-o", out, *SOURCES]
+
+
 def build(verbose: bool = False) -> str:
-    """Compiles the library (always) and returns its path."""
+    """Compiles the library (always) and returns its path.
+
+    Each translation unit is compiled to an object of its own, in parallel, and the objects are linked
+    into the shared library; no device code crosses translation units.
+    """
     os.makedirs(LIB_DIR, exist_ok=True)
-    tmp = LIB_PATH + ".tmp"
-    cmd = compile_command(tmp)
+    obj_dir = os.path.join(LIB_DIR, "obj")
+    os.makedirs(obj_dir, exist_ok=True)
+    objs = [os.path.join(obj_dir, os.path.splitext(os.path.basename(s))[0] + ".o") for s in SOURCES]
+    cmds = [[hipcc(), *_flags(), "-c", "-o", o, s] for s, o in zip(SOURCES, objs)]
     if verbose:
-        print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
+        for c in cmds:
+            print(" ".join(c), flush=True)
+    with ThreadPoolExecutor(max_workers=min(len(cmds), max(1, (os.cpu_count() or 1)))) as ex:
+        results = list(ex.map(lambda c: subprocess.run(c, capture_output=True, text=True), cmds))
+    for c, res in zip(cmds, results):
+        if res.returncode != 0:
+            raise RuntimeError(f"hipcc failed ({res.returncode}) on {c[-1]}:\n{res.stderr[-4000:]}")
+    tmp = LIB_PATH + ".tmp"
+    link = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    if verbose:
+        print(" ".join(link), flush=True)
+    res = subprocess.run(link, capture_output=True, text=True)
     if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed ({res.returncode}):\n{res.stderr[-4000:]}")
+        raise RuntimeError(f"hipcc link failed ({res.returncode}):\n{res.stderr[-4000:]}")
     os.replace(tmp, LIB_PATH)
     return LIB_PATH
 

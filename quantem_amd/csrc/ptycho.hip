@@ -968,7 +968,18 @@ Layout layout_of(const qem_ptycho_desc& d) {
   return L;
 }
 
+// The translation units that share qem::Params agree on its layout (see params_signature_local).
+int check_params_layout() {
+  static const bool same = qem::params_signature_local() == qem::params_signature_256();
+  if (!same)
+    return fail(QEM_ERR_LAUNCH,
+                "library build error: qem::Params has a different layout in ptycho.hip and ptycho256.hip "
+                "(translation units compiled from different headers); rebuild the library");
+  return QEM_OK;
+}
+
 int check_desc(const qem_ptycho_desc* d) {
+  if (int rc = check_params_layout()) return rc;
   if (d == nullptr) return fail(QEM_ERR_NULL, "desc is NULL");
   if (d->roi != 32 && d->roi != 64 && d->roi != 128 && d->roi != 256)
     return fail(QEM_ERR_UNSUPPORTED, "roi %d not supported (32, 64, 128 or 256)", d->roi);

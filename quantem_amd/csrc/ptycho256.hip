@@ -571,6 +571,8 @@ __global__ __launch_bounds__(T) void k_center256(int P, const float* __restrict_
 
 }  // namespace n256
 
+unsigned long long params_signature_256() { return params_signature_local(); }
+
 int launch_chain256_forward(const Params& p, hipStream_t st) {
   // M probe-spectrum blocks, then the blocks of the (B, 2, N) shift-ramp table
   const int ramp_blocks = (p.B * 2 * n256::N + n256::T - 1) / n256::T;

@@ -9,6 +9,9 @@
 namespace qem {
 
 struct Params {
+#ifdef QEM_PARAMS_LAYOUT_TEST
+  int layout_test_extra;  // only in the CPU test build that checks the layout comparison below
+#endif
   int S, M, B, H, W, P, loss_type;
   float mean_I;
   unsigned flags;
@@ -47,6 +50,28 @@ struct Params {
   int fwd_mode;  // FWD_* below
   cf* ckpt;      // (B*M*nck, N, N)
 };
+
+// Layout signature of Params as compiled in the current translation unit: its size and every field's
+// offset folded into one value.  Params crosses from ptycho.hip (the C ABI) into ptycho256.hip (the
+// N = 256 launchers) by reference, so both must agree on it; a build that compiled the two with
+// different versions of this header would hand the 256 kernels a struct whose pointers sit at other
+// offsets.  The entry points compare params_signature_local() with params_signature_256() and refuse
+// to run on a mismatch.
+#define QEM_PARAMS_FIELDS(QF)                                                                          \
+  QF(S) QF(M) QF(B) QF(H) QF(W) QF(P) QF(loss_type) QF(mean_I) QF(flags) QF(obj) QF(probe) QF(pos) QF(prop)        \
+  QF(targets) QF(mask) QF(descan) QF(loss_out) QF(intens_out) QF(dloss) QF(grad_obj) QF(grad_probe)           \
+  QF(grad_pos) QF(grad_descan) QF(psi) QF(scratch) QF(X) QF(gI) QF(loss_part) QF(pk) QF(gsum) QF(ramps)          \
+  QF(gpos_part) QF(gdesc_part) QF(s0) QF(s1) QF(wslots) QF(ck) QF(nck) QF(fwd_mode) QF(ckpt)
+// internal linkage: each translation unit evaluates its own copy (an inline function would be merged)
+static constexpr unsigned long long params_signature_local() {
+  unsigned long long h = sizeof(Params);
+#define QEM_PARAMS_FOLD(f) h = h * 1000003ull ^ (unsigned long long)__builtin_offsetof(Params, f);
+  QEM_PARAMS_FIELDS(QEM_PARAMS_FOLD)
+#undef QEM_PARAMS_FOLD
+  return h;
+}
+// params_signature_local() as evaluated in ptycho256.hip.
+unsigned long long params_signature_256();
 
 // forward launch modes
 enum : int {

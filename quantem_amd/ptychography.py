@@ -103,7 +103,9 @@ class MultisliceLoss(torch.autograd.Function):
     Inputs follow the layouts of the reference tensors (see ``include/qem_ptycho.h``).  Gradients
     are returned for ``obj``, ``probe``, ``positions`` and ``descan`` (the last two only when they
     require grad); propagators, targets and mask get ``None`` as in the reference, where they are
-    not optimised.
+    not optimised.  The backward pass consumes the workspace of its forward: differentiating the
+    same graph a second time (``retain_graph=True``) raises ``RuntimeError`` instead of returning
+    wrong gradients, and the backward itself is not differentiable.
     """
 
     @staticmethod
@@ -162,13 +164,22 @@ class MultisliceLoss(torch.autograd.Function):
         if need_grad:
             ctx.save_for_backward(obj_c, probe_c, pos_c, prop_c, desc_c, ws)
             ctx.desc = d
+        ctx.consumed = False
         if intens is None:
             intens = torch.empty((0,), dtype=torch.float32, device=device)
         ctx.mark_non_differentiable(intens)
         return loss, intens
 
     @staticmethod
+    @torch.autograd.function.once_differentiable
     def backward(ctx, grad_loss, grad_intens):
+        # The adjoint overwrites the workspace the forward filled (gradient patches over the stored
+        # waves, probe-gradient spectra over the far field), so a graph kept with retain_graph=True
+        # can be differentiated once only.
+        if ctx.consumed:
+            raise RuntimeError("multislice_loss: the forward's workspace was consumed by an earlier backward "
+                               "pass; run the forward again before differentiating a second time")
+        ctx.consumed = True
         obj_c, probe_c, pos_c, prop_c, desc_c, ws = ctx.saved_tensors
         d = ctx.desc
         device = obj_c.device

@@ -666,7 +666,8 @@ def main(argv=None) -> int:
     true_phase = prob["true_phase"].sum(0)
     crop = scan_window(spec)
     rec = Reconstruction(prob["positions"], amps, prob["propagators"], prob["probe"],
-                         (spec.num_slices,) + tuple(spec.obj_shape), mean_I, upsample_factor=spec.scan_step,
+                         (spec.num_slices,) + tuple(spec.obj_shape), mean_I,
+                         upsample_factor=synth.spec_upsample_factor(spec),
                          device=dev)
     nmse_hist = []
 

@@ -201,6 +201,21 @@ def scan_origin(spec: ProblemSpec) -> tuple:
     return tuple((np.asarray(spec.obj_shape) - ext) / 2.0)
 
 
+def upsample_factor(scan: Sequence[int], scan_step_px: float, sampling: float) -> float:
+    """The reference's ``upsample_factor`` (``dataset_models.py:451-456, 877-879``): the cropped object
+    shape floor(fov / sampling), rounded up to even, over the scan grid, averaged over the two axes,
+    with fov = scan step (Angstrom) * (gpts - 1).  Scan step given in object pixels here."""
+    gpts = np.asarray(scan, dtype=np.float64)
+    fov = (scan_step_px * sampling) * (gpts - 1)
+    shp = np.floor(fov / sampling)
+    shp += shp % 2
+    return float((shp.astype("int") / gpts).mean())
+
+
+def spec_upsample_factor(spec: "ProblemSpec") -> float:
+    return upsample_factor(spec.scan, spec.scan_step, spec.sampling)
+
+
 def make_problem(spec: ProblemSpec, seed: int = 0, with_data: bool = True) -> dict:
     """Object, probe, positions and propagators (and Poisson data computed by `intensity_fn`).
 

@@ -343,7 +343,8 @@ def evaluate(net: WarmStartNet, norm: FeatureNorm, sim: PairSimulator, seed: int
 
     def run(obj_init, t_offset):
         rec = Reconstruction(d["positions"], d["amplitudes"], sim.props_np, sim.probe_np, shape, d["mean_intensity"],
-                             upsample_factor=s.step, obj_init=obj_init, device=sim.dev)
+                             upsample_factor=synth.upsample_factor((s.scan, s.scan), s.step, s.sampling),
+                             obj_init=obj_init, device=sim.dev)
         curve = []
 
         def cb(ep, loss, r):
