@@ -51,6 +51,8 @@ EXPORTED_SYMBOLS = (
     "qem_conv3x3_wgrad_workspace_bytes",
     "qem_conv3x3_wgrad",
     "qem_adam_step_f32",
+    "qem_adamw_step_f32",
+    "qem_sgd_step_f32",
     "qem_object_constraints_workspace_bytes",
     "qem_object_constraints_forward",
     "qem_object_constraints_backward",
@@ -113,6 +115,11 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.qem_conv3x3_wgrad.restype = ctypes.c_int
     lib.qem_adam_step_f32.argtypes = [ctypes.c_int64, _vp, _vp, _vp, _vp] + [ctypes.c_double] * 6 + [i32, _vp]
     lib.qem_adam_step_f32.restype = ctypes.c_int
+    lib.qem_adamw_step_f32.argtypes = lib.qem_adam_step_f32.argtypes
+    lib.qem_adamw_step_f32.restype = ctypes.c_int
+    lib.qem_sgd_step_f32.argtypes = [ctypes.c_int64, _vp, _vp, _vp] + [ctypes.c_double] * 4 + [i32, ctypes.c_double,
+                                                                                              i32, _vp]
+    lib.qem_sgd_step_f32.restype = ctypes.c_int
     lib.qem_object_constraints_workspace_bytes.argtypes = [ctypes.c_int64, ctypes.POINTER(ctypes.c_size_t)]
     lib.qem_object_constraints_workspace_bytes.restype = ctypes.c_int
     lib.qem_object_constraints_forward.argtypes = [ctypes.c_int64, i32, _vp, _vp, _vp, ctypes.c_size_t, _vp]

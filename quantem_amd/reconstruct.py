@@ -22,7 +22,7 @@ import numpy as np
 import torch
 
 from .constraints import object_constraints
-from .optim import Adam, make_scheduler, step_scheduler
+from .optim import make_optimizer, make_scheduler, step_scheduler
 from .ptychography import analytical_gradients, multislice_loss
 
 
@@ -438,7 +438,7 @@ class Reconstruction:
                     learn_positions: bool = True, learn_descan: bool = True,
                     dataset_constraints: Optional[dict] = None, val_ratio: float = 0.0, val_mode: str = "grid",
                     schedulers: Optional[dict] = None, object_constraints: Optional[dict] = None,
-                    probe_constraints: Optional[dict] = None,
+                    probe_constraints: Optional[dict] = None, optimizers: Optional[dict] = None,
                     callback: Optional[Callable[[int, float, "Reconstruction"], None]] = None) -> list:
         """Runs `num_iters` epochs; returns the per-epoch mean batch loss (consistency + soft constraints).
 
@@ -456,8 +456,11 @@ class Reconstruction:
         ``val_ratio`` > 0 holds out scan positions (:func:`validation_split`): they take no optimizer
         step, and their mean batch loss after each epoch goes to ``self.val_history``.
 
-        Each model ("object", "probe", "dataset") has its own Adam, as in the reference
-        (``ptychography_opt.py:13-234``); ``schedulers`` maps a model name to the reference's scheduler
+        Each model ("object", "probe", "dataset") has its own optimizer, as in the reference
+        (``ptychography_opt.py:13-234``): Adam by default, or what ``optimizers`` names for it
+        (``{"object": {"type": "adamw", "weight_decay": 1e-2}}``; types "adam", "adamw", "sgd" with their
+        torch keyword arguments, ``optimizer_mixin.py:84-126``, on the HIP optimizer steps of
+        :mod:`quantem_amd.optim`; the rates above are the defaults of ``lr``); ``schedulers`` maps a model name to the reference's scheduler
         parameters (:func:`quantem_amd.optim.make_scheduler`, built with ``num_iter = num_iters``),
         stepped once per epoch with the epoch's mean loss (``ptychography.py:278-279``).  The learning
         rate of every model after each epoch goes to ``self.lr_history``.
@@ -490,9 +493,14 @@ class Reconstruction:
         # the object-gradient rescale of the autograd path (ptychography.py:330-332) is applied inside
         # the optimizer step; the analytical update is not rescaled, as in the reference
         scale = self.upsample_factor ** 2 / 2 if autograd else 1.0
-        opts = {"object": Adam([{"params": [self._obj], "lr": obj_lr, "grad_scale": scale}])}
+        opt_cfg = dict(optimizers or {})
+        unknown = set(opt_cfg) - {"object", "probe", "dataset"}
+        if unknown:
+            raise KeyError(f"optimizers for unknown models {sorted(unknown)}; known: object, probe, dataset")
+        opts = {"object": make_optimizer([{"params": [self._obj], "grad_scale": scale}], opt_cfg.get("object"),
+                                         obj_lr)}
         if learn_probe:
-            opts["probe"] = Adam([{"params": [self._probe], "lr": probe_lr}])
+            opts["probe"] = make_optimizer([{"params": [self._probe]}], opt_cfg.get("probe"), probe_lr)
         self._probe.requires_grad_(learn_probe)
         dset_params = []
         if learn_dset and learn_descan:
@@ -502,7 +510,7 @@ class Reconstruction:
         for t in (self.positions, self.descan):
             t.requires_grad_(any(t is q for q in dset_params))
         if dset_params:
-            opts["dataset"] = Adam([{"params": dset_params, "lr": float(dataset_lr)}])
+            opts["dataset"] = make_optimizer([{"params": dset_params}], opt_cfg.get("dataset"), float(dataset_lr))
         sched_cfg = dict(schedulers or {})
         unknown = set(sched_cfg) - {"object", "probe", "dataset"}
         if unknown:

@@ -137,13 +137,21 @@ class ConvBlock(torch.nn.Module):
 
 
 class WarmStartNet(torch.nn.Module):
-    """4-level U-Net (32-256 channels) on the scan grid + 2 upsampling stages to 4 px per scan step."""
+    """4-level U-Net (32-256 channels) over the object grid at one cell per scan step, then 2 upsampling
+    stages to the object's pixels (4 px per scan step).
 
-    def __init__(self, in_ch: int = 8, base: int = 32):
+    The (8, Sy, Sx) feature image of the scan is zero-padded by ``pad`` cells on every side to the
+    object's extent, with a ninth channel marking the scanned cells, so the output covers the whole
+    object ((Sy + 2 pad) * 4 px square: 256 x 256 for the 32 x 32 scan of a 256^2 object).  Outside the
+    illuminated area the data carry no information and the trained network returns a smooth mean there.
+    """
+
+    def __init__(self, in_ch: int = 8, base: int = 32, pad: int = 16):
         super().__init__()
+        self.pad = int(pad)
         c = [base, base * 2, base * 4, base * 8]
         self.enc = torch.nn.ModuleList()
-        prev = in_ch
+        prev = in_ch + 1  # + coverage channel
         for ch in c:
             self.enc.append(torch.nn.Sequential(ConvBlock(prev, ch), ConvBlock(ch, ch)))
             prev = ch
@@ -155,7 +163,11 @@ class WarmStartNet(torch.nn.Module):
         self.out = ConvBlock(16, 16, out_f32=True, relu=False)  # channel 0 is the phase
 
     def forward(self, feats: torch.Tensor) -> torch.Tensor:
-        x = feats
+        p = self.pad
+        cov = torch.ones_like(feats[:, :1])
+        x = F.pad(torch.cat([feats, cov], dim=1), (p, p, p, p))
+        if x.shape[-1] % 8 or x.shape[-2] % 8:
+            raise ValueError(f"padded feature grid {tuple(x.shape[-2:])} must be a multiple of 8 cells")
         skips = []
         for i, blk in enumerate(self.enc):
             x = blk(x)
@@ -207,8 +219,19 @@ class WarmStartSpec:
 
     @property
     def window(self) -> tuple:
+        """Object rows (= columns) of the scanned window, one scan step per scan point: the NMSE crop."""
         r0 = int(round(self.origin - self.step / 2))
         return (r0, r0 + self.fov)
+
+    @property
+    def pad_cells(self) -> int:
+        """Cells the network pads the scan grid with so that its output covers the whole object."""
+        pad = (self.obj_size / self.step - self.scan) / 2
+        w0 = self.window[0]
+        if pad != int(pad) or w0 != int(pad) * self.step:
+            raise ValueError("the scan grid must sit on whole cells of the object (obj_size / step - scan even, "
+                             "window starting at pad * step)")
+        return int(pad)
 
 
 def atom_phase_fft(size: int, n_atoms: int, sigma: float, amp_range, rng: np.random.Generator) -> np.ndarray:
@@ -265,8 +288,7 @@ class PairSimulator:
         feats = pattern_features(counts, self.edges, amplitudes=False)
         s = self.spec
         f_img = feats.view(s.scan, s.scan, 8).permute(2, 0, 1).contiguous()
-        w0, w1 = s.window
-        target = phases.sum(0)[w0:w1, w0:w1]
+        target = phases.sum(0)  # projected phase of the whole object
         return {"features": f_img, "target": torch.from_numpy(target.astype(np.float32)).to(self.dev),
                 "amplitudes": torch.sqrt(counts), "positions": pos, "phases": phases,
                 "mean_intensity": float(counts.sum(dim=(-2, -1)).mean())}
@@ -320,7 +342,8 @@ def train(net: WarmStartNet, feats: torch.Tensor, targets: torch.Tensor, norm: F
     return hist
 
 
-def predict_window(net: WarmStartNet, norm: FeatureNorm, feats: torch.Tensor) -> torch.Tensor:
+def predict_phase(net: WarmStartNet, norm: FeatureNorm, feats: torch.Tensor) -> torch.Tensor:
+    """The network's projected phase of the whole object from one (8, Sy, Sx) feature image."""
     with torch.no_grad():
         return net(norm(feats[None]))[0].float()
 
@@ -363,9 +386,8 @@ def evaluate(net: WarmStartNet, norm: FeatureNorm, sim: PairSimulator, seed: int
     t0 = time.perf_counter()
     counts = d["amplitudes"] ** 2
     f = pattern_features(counts, sim.edges, amplitudes=False).view(s.scan, s.scan, 8).permute(2, 0, 1)
-    phi = predict_window(net, norm, f)
-    init = torch.ones(shape, dtype=torch.complex64, device=sim.dev)
-    init[:, w0:w1, w0:w1] = torch.exp(1j * (phi / s.num_slices)).to(torch.complex64)
+    phi = predict_phase(net, norm, f)
+    init = torch.exp(1j * (phi / s.num_slices)).to(torch.complex64).expand(shape).contiguous()
     torch.cuda.synchronize()
     t_init = time.perf_counter() - t0
     warm = run(init, t_init)
@@ -404,7 +426,7 @@ def main(argv=None) -> int:
     log(f"simulated {args.train} pairs in {t_sim:.1f} s")
     norm = FeatureNorm(feats)
     torch.manual_seed(0)
-    net = WarmStartNet().to(dev)
+    net = WarmStartNet(pad=spec.pad_cells).to(dev)
     t0 = time.perf_counter()
     hist = train(net, feats, targets, norm, epochs=args.epochs, batch=args.batch, lr=args.lr, log=log)
     torch.cuda.synchronize()

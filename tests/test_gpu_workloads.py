@@ -1,0 +1,142 @@
+"""The named workloads at their full shapes, against the FP64 oracle on a few patterns and through
+size-independent properties on full batches.
+
+  headline   8 slices of 384^2 (2 A), 128^2 patterns, 4 modes (0.7/0.15/0.1/0.05), 64 x 64 scan
+  large      30 slices of 1024^2 (1.5 A), 256^2 patterns, 8 modes, 128 x 128 scan, batch 512
+
+Tolerances (relative L2 against the float64 oracle on the same complex64 inputs): intensities and loss
+1e-5; gradients 1e-4, or twice the error of the reference's own op order evaluated in complex64
+(oracle/torch_ref.py) where float32 itself departs further from float64 (deep stacks of slices).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import ptycho_oracle as po
+from oracle import torch_ref
+from quantem_amd import ptychography as qp
+from quantem_amd import synth
+from tests._cases import rel
+
+pytestmark = pytest.mark.gpu
+
+FWD_TOL = 1e-5
+GRAD_TOL = 1e-4
+
+
+def _t(x, dev):
+    return None if x is None else torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+
+
+def _workload_case(spec, idx, seed=0):
+    """Object, probe, propagators of the named workload and `idx` of its scan positions, with Poisson
+    amplitudes of the oracle's intensities (the workload's dose) as targets."""
+    prob = synth.make_problem(spec, seed=seed)
+    pos = prob["positions"][idx]
+    fw = po.forward(prob["obj"], prob["probe"], pos, prob["propagators"])
+    rng = np.random.default_rng(seed + 1)
+    counts = rng.poisson(np.clip(fw["I"], 0, None)).astype(np.float64)
+    P = prob["positions"].shape[0]
+    return {"obj": prob["obj"], "probe": prob["probe"], "positions": pos, "propagators": prob["propagators"],
+            "targets": np.sqrt(counts).astype(np.float32), "mask": None, "num_gpts": P,
+            "mean_intensity": float(counts.sum(axis=(-2, -1)).mean()), "I_oracle": fw["I"], "all_positions":
+            prob["positions"]}
+
+
+def _run(case, dev, segment_slices=0):
+    obj = _t(case["obj"], dev).requires_grad_(True)
+    probe = _t(case["probe"], dev).requires_grad_(True)
+    pos = _t(case["positions"], dev).requires_grad_(True)
+    loss, intens = qp.multislice_loss(obj, probe, pos, _t(case["propagators"], dev), _t(case["targets"], dev), None,
+                                      None, num_gpts=case["num_gpts"], mean_intensity=case["mean_intensity"],
+                                      return_intensities=True, segment_slices=segment_slices)
+    loss.backward()
+    return {"loss": loss.item(), "I": intens.cpu().numpy(), "obj": obj.grad.cpu().numpy(),
+            "probe": probe.grad.cpu().numpy(), "positions": pos.grad.cpu().numpy()}
+
+
+def _oracle(case):
+    return po.adjoint(case["obj"], case["probe"], case["positions"], case["propagators"], case["targets"], None,
+                      case["num_gpts"], case["mean_intensity"])
+
+
+def _complex64_reference(case):
+    return torch_ref.loss_and_grads(case["obj"], case["probe"], case["positions"], case["propagators"],
+                                    case["targets"], None, case["num_gpts"], case["mean_intensity"],
+                                    dtype=torch.complex64)
+
+
+def test_headline_gradients_match_oracle(cuda):
+    """Eight patterns spread over the headline scan (corners, edges, centre) on the full 8 x 384^2 object
+    with its 4 probe modes: intensities, loss and the object, probe and position gradients against the
+    FP64 oracle."""
+    idx = np.array([0, 63, 517, 2015, 2080, 3071, 4032, 4095])
+    print("headline: 8 patterns vs the FP64 oracle", flush=True)
+    case = _workload_case(synth.HEADLINE, idx)
+    got = _run(case, cuda)
+    ref = _oracle(case)
+    errs = {k: rel(got[k], ref[k]) for k in ("I", "obj", "probe", "positions")}
+    errs["loss"] = abs(got["loss"] - ref["loss"]) / abs(ref["loss"])
+    print("  " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
+    assert errs["I"] < FWD_TOL and errs["loss"] < FWD_TOL
+    for k in ("obj", "probe", "positions"):
+        assert errs[k] < GRAD_TOL, k
+
+
+@pytest.mark.parametrize("segment", [0, 6])
+def test_large_workload_matches_oracle(cuda, segment):
+    """Three patterns of the 30 x 1024^2, 8-mode, 256^2 workload, every incident wave stored (segment 0)
+    or 6-slice recomputation segments, against the FP64 oracle; the recomputed gradients equal the
+    stored-wave ones."""
+    idx = np.array([0, 8256, 16383])
+    print(f"large: 3 patterns vs the FP64 oracle, segment_slices={segment}", flush=True)
+    case = _workload_case(synth.LARGE, idx)
+    got = _run(case, cuda, segment_slices=segment)
+    ref = _oracle(case)
+    c64 = _complex64_reference(case)
+    assert rel(got["I"], ref["I"]) < FWD_TOL
+    assert abs(got["loss"] - ref["loss"]) / abs(ref["loss"]) < FWD_TOL
+    for k in ("obj", "probe", "positions"):
+        tol = max(GRAD_TOL, 2.0 * rel(c64[k], ref[k]))
+        print(f"  {k}: kernel {rel(got[k], ref[k]):.2e}, complex64 reference {rel(c64[k], ref[k]):.2e}", flush=True)
+        assert rel(got[k], ref[k]) < tol, k
+    if segment:
+        full = _run(case, cuda, segment_slices=0)
+        assert np.array_equal(full["I"], got["I"])
+        for k in ("obj", "probe", "positions"):
+            assert rel(got[k], full[k]) < 1e-6, k
+
+
+@pytest.mark.parametrize("segment", [0, 6])
+def test_large_batch_512_is_the_sum_of_its_parts(cuda, segment):
+    """The large workload's full 512-pattern launch: per-pattern intensities are bitwise those of four
+    128-pattern launches, loss and gradients are the sub-batch sums (the loss carries P / B, hence the
+    factor B_sub / B)."""
+    print(f"large: batch 512 decomposition, segment_slices={segment}", flush=True)
+    spec = synth.LARGE
+    prob = synth.make_problem(spec, seed=3)
+    dev = cuda
+    obj_t, probe_t = _t(prob["obj"], dev), _t(prob["probe"], dev)
+    props = _t(prob["propagators"], dev)
+    pos_t = _t(prob["positions"][4096:4096 + 512], dev)
+    tg = qp.forward_intensities(obj_t * 0.98, probe_t, pos_t, props).clamp_min(0).sqrt()
+    args = dict(num_gpts=16384, mean_intensity=float((tg ** 2).sum(dim=(-2, -1)).mean()), segment_slices=segment)
+
+    def run(sl):
+        o = obj_t.clone().requires_grad_(True)
+        p = probe_t.clone().requires_grad_(True)
+        q = pos_t[sl].clone().requires_grad_(True)
+        loss, I = qp.multislice_loss(o, p, q, props, tg[sl], None, None, return_intensities=True, **args)
+        loss.backward()
+        return loss.detach(), I, o.grad, p.grad, q.grad
+
+    full = run(slice(0, 512))
+    parts = [run(slice(k, k + 128)) for k in range(0, 512, 128)]
+    f = 128 / 512
+    assert torch.equal(full[1], torch.cat([pt[1] for pt in parts]))
+    assert abs(float(full[0]) - f * sum(float(pt[0]) for pt in parts)) < 1e-5 * abs(float(full[0]))
+    for i in (2, 3):
+        ref = f * sum(pt[i] for pt in parts)
+        assert rel(full[i].cpu().numpy(), ref.cpu().numpy()) < 1e-5, i
+    assert rel(full[4].cpu().numpy(), (f * torch.cat([pt[4] for pt in parts])).cpu().numpy()) < 1e-5

@@ -120,7 +120,7 @@ def test_reconstruction_reduces_loss_and_error(cuda):
     I = forward_intensities(obj, probe, pos, None).cpu().numpy()
     amps, mean_I = synth.poisson_amplitudes(I, np.random.default_rng(2))
     rec = rc.Reconstruction(prob["positions"], amps, None, prob["probe"], (1,) + spec.obj_shape, mean_I,
-                            upsample_factor=spec.scan_step, device=dev)
+                            upsample_factor=synth.spec_upsample_factor(spec), device=dev)
     hist = rec.reconstruct(10, 32, obj_lr=2e-2, learn_probe=False, seed=0)
     assert hist[-1] < 0.7 * hist[0]
     est = rc.projected_phase(rec.obj_constrained.detach()).cpu().numpy()
@@ -143,7 +143,7 @@ def test_analytical_reconstruction_reduces_loss(cuda):
     I = forward_intensities(obj, probe, pos, None).cpu().numpy()
     amps, mean_I = synth.poisson_amplitudes(I, np.random.default_rng(2))
     rec = rc.Reconstruction(prob["positions"], amps, None, prob["probe"], (1,) + spec.obj_shape, mean_I,
-                            upsample_factor=spec.scan_step, device=cuda)
+                            upsample_factor=synth.spec_upsample_factor(spec), device=cuda)
     hist = rec.reconstruct(8, 32, obj_lr=2e-2, learn_probe=False, seed=0, autograd=False)
     assert np.isfinite(hist).all()
     assert hist[-1] < 0.8 * hist[0]
@@ -165,7 +165,7 @@ def test_single_slice_reference_case(cuda):
                             torch.from_numpy(prob["positions"]).to(cuda), None).cpu().numpy()
     amps, mean_I = synth.poisson_amplitudes(I, np.random.default_rng(1))
     rec = rc.Reconstruction(prob["positions"], amps, None, prob["probe"], (1,) + spec.obj_shape, mean_I,
-                            upsample_factor=spec.scan_step, device=cuda)
+                            upsample_factor=synth.spec_upsample_factor(spec), device=cuda)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     hist = rec.reconstruct(20, spec.batch, seed=0)
@@ -210,7 +210,7 @@ def test_position_refinement_recovers_perturbed_positions(cuda):
     rng = np.random.default_rng(5)
     start = (true + rng.uniform(-1.5, 1.5, size=true.shape)).astype(np.float32)
     rec = rc.Reconstruction(start, amps, None, prob["probe"], (1,) + spec.obj_shape, mean_I,
-                            upsample_factor=spec.scan_step, obj_init=prob["obj"], device=cuda)
+                            upsample_factor=synth.spec_upsample_factor(spec), obj_init=prob["obj"], device=cuda)
     err0 = float(np.sqrt(np.mean((start - true) ** 2)))
     hist = rec.reconstruct(20, 32, obj_lr=0.0, learn_probe=False, seed=0, dataset_lr=2e-2, learn_descan=False)
     est = rec.positions.detach().cpu().numpy()
@@ -230,7 +230,7 @@ def test_descan_refinement_and_validation_loss(cuda):
     ramp = np.stack([np.linspace(-1.0, 1.0, n), np.linspace(0.8, -0.8, n)], axis=1).astype(np.float32)
     spec, prob, amps, mean_I = _small_problem(cuda, ramp)
     rec = rc.Reconstruction(prob["positions"], amps, None, prob["probe"], (1,) + spec.obj_shape, mean_I,
-                            upsample_factor=spec.scan_step, obj_init=prob["obj"], device=cuda)
+                            upsample_factor=synth.spec_upsample_factor(spec), obj_init=prob["obj"], device=cuda)
     hist = rec.reconstruct(15, 32, obj_lr=0.0, learn_probe=False, seed=0, dataset_lr=2e-2, learn_positions=False,
                            val_ratio=0.25, dataset_constraints={"descan_tv_weight": 1e-3})
     d = rec.descan.detach().cpu().numpy()
@@ -249,7 +249,7 @@ def test_schedulers_drive_the_per_model_rates(cuda):
     """An exponential schedule on the object and a plateau schedule on the probe act per epoch."""
     spec, prob, amps, mean_I = _small_problem(cuda)
     rec = rc.Reconstruction(prob["positions"], amps, None, prob["probe"], (1,) + spec.obj_shape, mean_I,
-                            upsample_factor=spec.scan_step, device=cuda)
+                            upsample_factor=synth.spec_upsample_factor(spec), device=cuda)
     hist = rec.reconstruct(3, 48, obj_lr=2e-2, probe_lr=1e-3, seed=0,
                            schedulers={"object": {"type": "exp", "gamma": 0.5}, "probe": {"type": "plateau"}})
     assert np.isfinite(hist).all()
@@ -257,6 +257,23 @@ def test_schedulers_drive_the_per_model_rates(cuda):
     assert rec.lr_history["probe"] == pytest.approx([1e-3] * 3)
     with pytest.raises(KeyError):
         rec.reconstruct(1, 48, schedulers={"positions": {"type": "exp"}})
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("opt", [{"type": "adamw"}, {"type": "sgd", "lr": 2.0, "momentum": 0.9},
+                                 {"type": "adam", "lr": 2e-2}])
+def test_optimizer_types_drive_the_loop(cuda, opt):
+    """The reference's optimizer types (optimizer_mixin.py:117-122) on the object through the loop:
+    each reduces the loss of the small single-slice problem."""
+    spec, prob, amps, mean_I = _small_problem(cuda)
+    rec = rc.Reconstruction(prob["positions"], amps, None, prob["probe"], (1,) + spec.obj_shape, mean_I,
+                            upsample_factor=synth.spec_upsample_factor(spec), device=cuda)
+    hist = rec.reconstruct(6, 32, obj_lr=2e-2, learn_probe=False, seed=0, optimizers={"object": opt})
+    print(f"optimizer {opt}: loss {hist[0]:.4g} -> {hist[-1]:.4g}", flush=True)
+    assert np.isfinite(hist).all()
+    assert hist[-1] < 0.9 * hist[0]
+    with pytest.raises(KeyError):
+        rec.reconstruct(1, 32, optimizers={"positions": {"type": "sgd"}})
 
 
 def test_soft_constraint_losses_known_answers():
@@ -302,7 +319,7 @@ def test_object_tv_constraint_smooths_the_reconstruction(cuda):
 
     def run(w):
         rec = rc.Reconstruction(prob["positions"], amps, None, prob["probe"], (1,) + spec.obj_shape, mean_I,
-                                upsample_factor=spec.scan_step, device=cuda)
+                                upsample_factor=synth.spec_upsample_factor(spec), device=cuda)
         hist = rec.reconstruct(6, 32, obj_lr=2e-2, learn_probe=False, seed=0,
                                object_constraints={"tv_weight_xy": w})
         ph = rec.obj_constrained.detach().angle()
@@ -411,7 +428,7 @@ def test_potential_object_reconstruction(cuda, autograd):
     """A real potential object (transmission exp(iV), positivity) reconstructs in both update modes."""
     spec, prob, amps, mean_I = _small_problem(cuda)
     rec = rc.Reconstruction(prob["positions"], amps, None, prob["probe"], (1,) + spec.obj_shape, mean_I,
-                            obj_type="potential", upsample_factor=spec.scan_step, device=cuda)
+                            obj_type="potential", upsample_factor=synth.spec_upsample_factor(spec), device=cuda)
     hist = rec.reconstruct(6, 32, obj_lr=2e-2, learn_probe=False, seed=0, autograd=autograd,
                            object_constraints={"tv_weight_xy": 1e-3})
     assert np.isfinite(hist).all() and hist[-1] < 0.9 * hist[0]
