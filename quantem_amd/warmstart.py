@@ -141,7 +141,8 @@ class WarmStartNet(torch.nn.Module):
     stages to the object's pixels (4 px per scan step).
 
     The (8, Sy, Sx) feature image of the scan is zero-padded by ``pad`` cells on every side to the
-    object's extent, with a ninth channel marking the scanned cells, so the output covers the whole
+    object's extent, with a ninth channel marking the scanned cells (and zero channels up to a multiple
+    of four, the MFMA convolution's channel granularity), so the output covers the whole
     object ((Sy + 2 pad) * 4 px square: 256 x 256 for the 32 x 32 scan of a 256^2 object).  Outside the
     illuminated area the data carry no information and the trained network returns a smooth mean there.
     """
@@ -151,7 +152,8 @@ class WarmStartNet(torch.nn.Module):
         self.pad = int(pad)
         c = [base, base * 2, base * 4, base * 8]
         self.enc = torch.nn.ModuleList()
-        prev = in_ch + 1  # + coverage channel
+        self.in_pad = (-(in_ch + 1)) % 4  # zero channels: the MFMA conv takes channel counts in fours
+        prev = in_ch + 1 + self.in_pad  # + coverage channel
         for ch in c:
             self.enc.append(torch.nn.Sequential(ConvBlock(prev, ch), ConvBlock(ch, ch)))
             prev = ch
@@ -165,7 +167,7 @@ class WarmStartNet(torch.nn.Module):
     def forward(self, feats: torch.Tensor) -> torch.Tensor:
         p = self.pad
         cov = torch.ones_like(feats[:, :1])
-        x = F.pad(torch.cat([feats, cov], dim=1), (p, p, p, p))
+        x = F.pad(torch.cat([feats, cov], dim=1), (p, p, p, p, 0, self.in_pad))
         if x.shape[-1] % 8 or x.shape[-2] % 8:
             raise ValueError(f"padded feature grid {tuple(x.shape[-2:])} must be a multiple of 8 cells")
         skips = []
