@@ -32,13 +32,10 @@ from quantem_amd import optim as qoptim  # noqa: E402
 from quantem_amd import synth  # noqa: E402
 
 METRIC = "patterns/s fwd+adjoint, % HBM roofline; time-to-target-NMSE warm-start vs flat"
-FP32_PEAK_TFLOPS = 157.3
+FP32_PEAK_TFLOPS = 157.3  # MI355X vector FP32 with packed (2-wide) FMA
+FP32_UNPACKED_TFLOPS = 78.6  # the same without packed FP32, which this library's device code is built without
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the adjoint kernel at this workload, from rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE runs of this script (profiles/multislice_8x384_128px_4modes_hbm_traffic.md): FETCH_SIZE
-# 722546 KiB x 2.0 + WRITE_SIZE 1324088 KiB x 1.0, the factors being what the same counters read for
-# torch.neg over 1 GiB (FETCH_SIZE 524311 KiB, WRITE_SIZE 1048576 KiB).
-ADJOINT_TRAFFIC_BYTES = (722546 * 2.0 + 1324088 * 1.0) * 1024
+CALIBRATION_BYTES = 1 << 30  # torch.neg over 1 GiB: 1 GiB read, 1 GiB written
 
 
 def log(msg: str) -> None:
@@ -87,11 +84,38 @@ def gpu_pass(state: dict, data: dict, idx_list: list, segment_slices: int = 0) -
         opt.step()  # object-gradient rescale (grad_scale of the object group) + Adam, one kernel per tensor
 
 
+def host_cpu() -> dict:
+    """Model name, physical cores and hardware threads of the host (from /proc/cpuinfo), and the
+    threads this process may run on."""
+    model, cores, threads = platform.processor() or platform.machine(), set(), 0
+    try:
+        phys = core = None
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "processor":
+                    threads += 1
+                elif k == "model name":
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                    cores.add((phys, core))
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return {"model": model, "physical_cores": len(cores) or None, "hw_threads": threads or None,
+            "usable_threads": usable}
+
+
 def cpu_baseline(data: dict, spec: synth.ProblemSpec, seconds: float, sample_batch: int) -> dict:
     """The reference op order on the host cores (complex64 torch, loss.backward()), bounded sample."""
     from oracle import torch_ref
 
-    threads = os.cpu_count() or 1
+    host = host_cpu()
+    threads = host["usable_threads"]
     torch.set_num_threads(threads)
     idx = torch.arange(sample_batch)
     obj = torch.ones_like(data["obj_true"]).cpu().requires_grad_(True)
@@ -115,10 +139,99 @@ def cpu_baseline(data: dict, spec: synth.ProblemSpec, seconds: float, sample_bat
         el = time.perf_counter() - t0
         if el >= seconds:
             break
-    return {"value": n * sample_batch / el, "unit": "patterns/s", "cores": threads, "kind": "port",
+    return {"value": n * sample_batch / el, "unit": "patterns/s", "cores": torch.get_num_threads(), "kind": "port",
             "sample": f"{n} batches x {sample_batch} patterns of the same workload (torch {torch.__version__} "
                       f"complex64 CPU, reference op order, loss.backward()), {el:.1f} s, "
-                      f"{torch.get_num_threads()} threads, {platform.processor() or platform.machine()}"}
+                      f"{torch.get_num_threads()} threads on {host['model']} ({host['physical_cores']} physical cores, "
+                      f"{host['hw_threads']} hardware threads on the host)",
+            "host": host}
+
+
+def counter_child(batches_to_run: int = 2) -> int:
+    """Profiled child of measure_traffic(): torch.neg over 1 GiB (the calibration kernel), then
+    `batches_to_run` forward + loss + backward batches of the headline workload's shapes."""
+    dev = torch.device("cuda:0")
+    _native.load()
+    src = torch.ones(CALIBRATION_BYTES // 4, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    del src, dst
+    spec = synth.HEADLINE
+    S, (H, W), N, M, B = spec.num_slices, spec.obj_shape, spec.roi, spec.num_modes, spec.batch
+    prob = synth.make_problem(spec, seed=0)
+    obj = torch.from_numpy(prob["obj"]).to(dev).requires_grad_(True)
+    probe = torch.from_numpy(prob["probe"]).to(dev).requires_grad_(True)
+    pos = torch.from_numpy(prob["positions"][:B]).to(dev)
+    props = torch.from_numpy(prob["propagators"]).to(dev)
+    tg = torch.rand((B, N, N), device=dev) * 10
+    for _ in range(batches_to_run):
+        loss = qp.multislice_loss(obj, probe, pos, props, tg, None, None, num_gpts=4096, mean_intensity=1e5)
+        loss.backward()
+    torch.cuda.synchronize()
+    return 0
+
+
+def measure_traffic(timeout_s: float = 150.0) -> dict | None:
+    """HBM bytes per dispatch of the multislice kernels at the headline shape, measured in this run:
+    two rocprofv3 passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) over a child process
+    (counter_child), each scaled by what the same counter reads for the 1 GiB calibration copy."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        return None
+    vals = {}
+    with tempfile.TemporaryDirectory(prefix="qem_counters_") as tmp:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            out = os.path.join(tmp, counter)
+            cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "run", "--",
+                   sys.executable, os.path.abspath(__file__), "--counter-child"]
+            try:
+                r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout_s, start_new_session=True,
+                                   cwd=tmp)
+            except subprocess.TimeoutExpired:
+                log(f"counter pass {counter} timed out")
+                return None
+            files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
+            if not files:
+                log(f"counter pass {counter}: no counter file (rc {r.returncode}): {r.stderr[-500:]}")
+                return None
+            per = {}
+            with open(files[0]) as f:
+                for row in csv.DictReader(f):
+                    if row.get("Counter_Name") != counter:
+                        continue
+                    name = row.get("Kernel_Name", "")
+                    per.setdefault(name, []).append(float(row["Counter_Value"]))
+            vals[counter] = per
+
+    def pick(per, *keys):
+        for name, v in per.items():
+            if all(k in name for k in keys):
+                return float(np.mean(v)), len(v)
+        return None, 0
+
+    cal_f, _ = pick(vals["FETCH_SIZE"], "neg")
+    cal_w, _ = pick(vals["WRITE_SIZE"], "neg")
+    if not cal_f or not cal_w:
+        return None
+    sf = CALIBRATION_BYTES / (cal_f * 1024.0)
+    sw = CALIBRATION_BYTES / (cal_w * 1024.0)
+    kernels = {"adjoint": ("k_adjoint<128",), "forward": ("k_forward<128", "true>"),
+               "object_reduce": ("k_obj_reduce",), "loss": ("k_loss<",), "probe_reduce": ("k_probe_reduce",)}
+    res = {"scale_fetch": round(sf, 4), "scale_write": round(sw, 4), "kernels": {}}
+    for k, keys in kernels.items():
+        f, nf = pick(vals["FETCH_SIZE"], *keys)
+        w, nw = pick(vals["WRITE_SIZE"], *keys)
+        if f is None or w is None:
+            continue
+        res["kernels"][k] = {"read": f * 1024.0 * sf, "write": w * 1024.0 * sw, "dispatches": nf}
+    return res
 
 
 def main() -> int:
@@ -127,7 +240,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--cpu-batch", type=int, default=8)
+    ap.add_argument("--cpu-batch", type=int, default=256, help="patterns per CPU batch (the GPU batch)")
+    ap.add_argument("--no-counters", action="store_true", help="skip the two rocprofv3 counter passes")
+    ap.add_argument("--counter-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--workload", default="headline", choices=sorted(synth.WORKLOADS),
@@ -141,6 +256,8 @@ def main() -> int:
                     help="run one elementwise kernel over 1 GiB first (torch.neg: 1 GiB read, 1 GiB written), "
                          "the reference point for the HBM byte counters")
     args = ap.parse_args()
+    if args.counter_child:
+        return counter_child()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
               file=sys.stderr, flush=True)
@@ -199,17 +316,35 @@ def main() -> int:
     t_adj = kt.adj_ms / max(kt.n_adj, 1)
     dom, t_dom = ("multislice adjoint kernel", t_adj) if t_adj >= t_fwd else ("multislice forward kernel", t_fwd)
     achieved = flops_per_launch / (t_dom / 1e3) / 1e12
-    # measured counters exist for the headline workload's adjoint kernel with stored waves
-    measured = args.workload == "headline" and args.segment == 0 and dom == "multislice adjoint kernel"
-    traffic = ADJOINT_TRAFFIC_BYTES if measured else None
-    roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+    # HBM bytes per launch from the counter passes (headline shapes, every wave stored)
+    counters = None
+    if not args.no_counters and args.workload == "headline" and args.segment == 0:
+        log("counter passes (rocprofv3 FETCH_SIZE, WRITE_SIZE)")
+        counters = measure_traffic()
+    key = "adjoint" if dom == "multislice adjoint kernel" else "forward"
+    kc = (counters or {}).get("kernels", {}).get(key)
+    traffic = (kc["read"] + kc["write"]) if kc else None
+    roof = {"bound": "vector-fp32", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-            "traffic": traffic,
+            "peak_unpacked": FP32_UNPACKED_TFLOPS, "frac_unpacked": round(achieved / FP32_UNPACKED_TFLOPS, 4),
+            "traffic": round(traffic) if traffic else None,
             "traffic_gbs": round(traffic / (t_dom / 1e3) / 1e9, 1) if traffic else None,
             "kernel": dom, "kernel_ms": round(t_dom, 4),
             "fwd_kernel_ms": round(t_fwd, 4), "adj_kernel_ms": round(t_adj, 4),
-            "note": "FP32 vector (FFT-flop) bound, no MFMA used; 157.3 TFLOP/s is the MI355X FP32 peak; "
-                    "algorithmic flops = B*M*2S*5N^2*log2(N^2) per launch"}
+            "note": "FFT-flop bound on the FP32 vector units (no dense contraction, no MFMA); peak 157.3 TFLOP/s "
+                    "assumes packed FP32, which this build disables, so 78.6 TFLOP/s is the reachable vector peak "
+                    "(frac_unpacked); algorithmic flops = B*M*2S*5N^2*log2(N^2) per launch; traffic = HBM bytes "
+                    "per launch of that kernel from rocprofv3 FETCH_SIZE / WRITE_SIZE measured in this run, each "
+                    "scaled by the counter's reading for a 1 GiB copy"}
+    if counters:
+        alg_batch = B * (24 * S * N * N + 4 * N * N) + 16 * M * N * N + 8 * (S - 1) * N * N + 8 * S * H * W
+        ks = counters["kernels"]
+        per_batch = sum(v["read"] + v["write"] for v in ks.values())
+        roof["counters"] = {k: {"read_GB": round(v["read"] / 1e9, 4), "write_GB": round(v["write"] / 1e9, 4)}
+                            for k, v in ks.items()}
+        roof["counter_scale"] = {"fetch": counters["scale_fetch"], "write": counters["scale_write"]}
+        roof["batch_traffic_GB"] = round(per_batch / 1e9, 3)
+        roof["batch_algorithmic_GB"] = round(alg_batch / 1e9, 3)
     cpu = None
     if not args.no_cpu:
         log(f"cpu baseline ~{args.cpu_seconds:.0f} s")

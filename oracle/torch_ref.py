@@ -112,3 +112,19 @@ def loss_and_grads(obj, probe, positions, propagators, targets, mask, num_gpts, 
     if d is not None:
         out["descan"] = d.grad.numpy()
     return out
+
+
+def orthogonalize_probe_mgs(probe: torch.Tensor) -> torch.Tensor:
+    """The reference's probe orthogonalisation (probe_models.py:484-525) in torch: modified Gram-Schmidt,
+    each mode rescaled to its input norm, sorted by intensity, differentiable by autograd."""
+    norms = torch.sqrt(torch.sum(probe.real.square() + probe.imag.square(), dim=(-2, -1), keepdim=True))
+    out = []
+    for i in range(probe.shape[0]):
+        p = probe[i]
+        for q in out:
+            p = p - torch.sum(q.conj() * p) * q
+        n = torch.sqrt(torch.sum(p.real.square() + p.imag.square())).clamp_min(1e-12)
+        out.append(p / n)
+    out = torch.stack(out) * norms.view(-1, 1, 1)
+    order = torch.argsort(torch.sum(torch.abs(out).square(), dim=(-2, -1)), descending=True)
+    return torch.complex(out.real[order], out.imag[order])

@@ -57,6 +57,9 @@ EXPORTED_SYMBOLS = (
     "qem_object_constraints_forward",
     "qem_object_constraints_backward",
     "qem_pattern_com",
+    "qem_probe_orthogonalize_workspace_bytes",
+    "qem_probe_orthogonalize_forward",
+    "qem_probe_orthogonalize_backward",
     "qem_center_amplitudes_workspace_bytes",
     "qem_center_amplitudes",
 )
@@ -132,6 +135,13 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.qem_center_amplitudes_workspace_bytes.restype = ctypes.c_int
     lib.qem_center_amplitudes.argtypes = [i32, i32, _vp, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]
     lib.qem_center_amplitudes.restype = ctypes.c_int
+    i64 = ctypes.c_int64
+    lib.qem_probe_orthogonalize_workspace_bytes.argtypes = [i32, i64, ctypes.POINTER(ctypes.c_size_t)]
+    lib.qem_probe_orthogonalize_workspace_bytes.restype = ctypes.c_int
+    lib.qem_probe_orthogonalize_forward.argtypes = [i32, i64, _vp, _vp, _vp, ctypes.c_size_t, _vp]
+    lib.qem_probe_orthogonalize_forward.restype = ctypes.c_int
+    lib.qem_probe_orthogonalize_backward.argtypes = [i32, i64, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]
+    lib.qem_probe_orthogonalize_backward.restype = ctypes.c_int
     lib.qem_abi_version.argtypes = []
     lib.qem_abi_version.restype = ctypes.c_int32
 

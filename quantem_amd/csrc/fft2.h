@@ -313,6 +313,14 @@ struct Fft2 {
   __device__ __forceinline__ int p1_off(int e) const { return p1base + (e / RC1) * NRP * N + (e % RC1) * RC2; }
   // Natural frequency row of register e in layout P3 (frequency column is kc).
   __device__ __forceinline__ int p3_row(int e) const { return p3k12[e / RR3] + RR1 * RR2 * (e % RR3); }
+  // Offset of register e in the stored-wave layout (per-slice waves, gradient patches, checkpoints):
+  // natural rows, each row's columns permuted so that registers 2k and 2k+1 of the RC2 threads that
+  // share a row sit side by side, thread after thread (column c = r1 * RC2 + c0 at position
+  // (r1 / 2) * 2 * RC2 + 2 * c0 + r1 % 2).  A 16-byte access of a register pair then covers whole
+  // 128-byte lines across the lanes (RC2 = 8 at N = 128) instead of 64-byte pieces of natural rows.
+  __device__ __forceinline__ int wv_off(int e) const {
+    return rp * N + 2 * c0 + (e / RC1) * NRP * N + ((e % RC1) / 2) * 2 * RC2 + (e % 2);
+  }
   // Offset kr*N + kc of register e in an N x N row-major array (P3).
   __device__ __forceinline__ int p3_off(int e) const { return p3base[e / RR3] + RR1 * RR2 * N * (e % RR3); }
 

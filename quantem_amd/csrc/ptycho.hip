@@ -115,7 +115,7 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
     // a segment of an earlier forward: start from its checkpoint wave
     const cf* src = p.ckpt + ((size_t)item * p.nck + p.s0 / p.ck) * N * N;
 #pragma unroll
-    for (int e = 0; e < PTS; ++e) v[e] = src[f.p1_off(e)];
+    for (int e = 0; e < PTS; e += 2) ld_stream2(src + f.wv_off(e), v[e], v[e + 1]);
     // through an empty asm, so that the values enter the slice loop as plain registers: as loads
     // still possibly in flight at the loop head they made every wave store of every slice wait
 #pragma unroll
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
     }
     if (ALL || dst != nullptr) {
 #pragma unroll
-      for (int e = 0; e < PTS; ++e) st_stream(dst + f.p1_off(e), v[e]);
+      for (int e = 0; e < PTS; e += 2) st_stream2(dst + f.wv_off(e), v[e], v[e + 1]);
     }
     if (last_only) break;
 #pragma unroll
@@ -345,7 +345,7 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
       const float fr = fftfreq1<N>(row), fc = fftfreq1<N>(col);
       const cf ramp = cmul(expm2pi(fr * ddr), expm2pi(fc * ddc));
       if (want_desc) {
-        const cf xo = cmul(cmul(obj_at(O, oo.at(e)), ps[f.p1_off(e)]), ramp);
+        const cf xo = cmul(cmul(obj_at(O, oo.at(e)), ps[f.wv_off(e)]), ramp);
         const float im = cconjmul(v[e], xo).y;
         ar += (double)(fr * im);
         ac += (double)(fc * im);
@@ -380,9 +380,10 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
     oo.opaque();
     if constexpr (AHEAD) {
 #pragma unroll
-      for (int i = 0; i < PTS; ++i) {
-        st_stream(ps + f.p1_off(i), cconjmul(pre[i], v[i]));
+      for (int i = 0; i < PTS; i += 2) {
+        st_stream2(ps + f.wv_off(i), cconjmul(pre[i], v[i]), cconjmul(pre[i + 1], v[i + 1]));
         v[i] = cconjmul(ob[i], v[i]);
+        v[i + 1] = cconjmul(ob[i + 1], v[i + 1]);
       }
     } else {
 #pragma unroll
@@ -390,15 +391,17 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
         cf t[CH], o[CH];
 #pragma unroll
         for (int i = 0; i < CH; ++i) {
-          if (e0 + i < PF && have_pre)
+          if (e0 + i < PF && have_pre) {
             t[i] = pre[e0 + i];
-          else
-            t[i] = ld_stream(ps + f.p1_off(e0 + i));
+          } else if (i % 2 == 0) {
+            ld_stream2(ps + f.wv_off(e0 + i), t[i], t[i + 1]);
+          }
           o[i] = obj_at(O, oo.at(e0 + i));
         }
 #pragma unroll
         for (int i = 0; i < CH; ++i) {
-          st_stream(ps + f.p1_off(e0 + i), cconjmul(t[i], v[e0 + i]));
+          if (i % 2 == 0)
+            st_stream2(ps + f.wv_off(e0 + i), cconjmul(t[i], v[e0 + i]), cconjmul(t[i + 1], v[e0 + i + 1]));
           v[e0 + i] = cconjmul(o[i], v[e0 + i]);
         }
       }
@@ -408,7 +411,7 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
     // the top slice's psi and object values, loaded as every later slice's are prefetched
     const cf* pt = p.psi + ((size_t)item * p.wslots + (p.s1 - 1 - p.s0)) * N * N;
 #pragma unroll
-    for (int i = 0; i < PF; ++i) pre[i] = ld_stream(pt + f.p1_off(i));
+    for (int i = 0; i < PF; i += 2) ld_stream2(pt + f.wv_off(i), pre[i], pre[i + 1]);
     oo.opaque();
 #pragma unroll
     for (int i = 0; i < PTS; ++i) ob[i] = obj_at(p.obj + (size_t)(p.s1 - 1) * HW, oo.at(i));
@@ -440,7 +443,7 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
         const int sl = s - 1 >= p.s0 ? s - 1 - p.s0 : 0;
         const cf* pn = p.psi + ((size_t)item * p.wslots + sl) * N * N;
 #pragma unroll
-        for (int i = 0; i < PF; ++i) pre[i] = ld_stream(pn + f.p1_off(i));
+        for (int i = 0; i < PF; i += 2) ld_stream2(pn + f.wv_off(i), pre[i], pre[i + 1]);
       }
       const cf* On = p.obj + (size_t)(s - 1) * HW;
       f.inv_hooked(v, lds, [&] {
@@ -508,6 +511,30 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
 }
 
 // ---------------------------------------------------------------------------------------------
+// Column permutation of the stored-wave layout (Fft2::wv_off) for N <= 128; the N = 256 chain stores
+// natural rows.  stored_pos: position of natural patch column c in a stored row; stored_col: inverse.
+template <int N>
+__device__ __forceinline__ int stored_pos(int c) {
+  if constexpr (N <= 128) {
+    constexpr int RC2 = Fft2<N>::RC2;
+    const int c0 = c % RC2, r1 = c / RC2;
+    return (r1 / 2) * 2 * RC2 + 2 * c0 + (r1 % 2);
+  } else {
+    return c;
+  }
+}
+template <int N>
+__device__ __forceinline__ int stored_col(int pos) {
+  if constexpr (N <= 128) {
+    constexpr int RC2 = Fft2<N>::RC2;
+    const int rem = pos % (2 * RC2);
+    return (2 * (pos / (2 * RC2)) + rem % 2) * RC2 + rem / 2;
+  } else {
+    return pos;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
 // Fourier projection residual of the analytical update (ptychography.py:356-372), over X:
 //   one mode:  X' = A exp(i angle(X))             -> X' - X
 //   M modes:   X'_m = (A / a) X_m, a = sqrt(sum_m |X_m + 1e-9|^2), A / a = 0 where a = 0
@@ -568,7 +595,7 @@ __global__ __launch_bounds__(256) void k_overlap(Params p, float* out) {
       int ux = sv0[i] + tx;
       if (ux >= p.W) ux -= p.W;
       if (x >= p.W || ux >= N) continue;
-      const int col = (ux + N / 2) & (N - 1);
+      const int col = stored_pos<N>((ux + N / 2) & (N - 1));
       const cf* g = p.psi + (((size_t)(b0 + i) * p.M + m) * p.wslots + s) * N * N + col;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
@@ -640,7 +667,7 @@ __global__ __launch_bounds__(256) void k_obj_reduce(Params p) {
       int ux = sv0[i] + tx;
       if (ux >= p.W) ux -= p.W;
       const bool xok = x < p.W && ux < N;
-      const int col = (ux + N / 2) & (N - 1);
+      const int col = stored_pos<N>((ux + N / 2) & (N - 1));
       const cf* g = p.psi + ((size_t)(b0 + i) * p.M * p.wslots + blockIdx.z) * N * N + col;
       const cf* rowp[4];
       bool ok[4];
@@ -686,7 +713,7 @@ __global__ __launch_bounds__(256) void k_obj_reduce(Params p) {
 
 // Deterministic object-gradient scatter-add, one object row per block (the layout used when the row
 // fits the LDS): the block lists, in batch order, the patterns whose patch covers the row; each of
-// its N threads (one per stored patch column) then reads that patch row of every mode (N * 8
+// its N threads (one per stored position of a patch row) then reads that patch row of every mode (N * 8
 // contiguous bytes per mode), sums the modes in order and adds the sum into a W-wide LDS row.  The
 // columns of one patch are distinct (W >= N), those of consecutive patches may coincide, so a barrier
 // separates the adds of consecutive patterns: every object pixel sums its patterns in increasing b.
@@ -699,7 +726,8 @@ __global__ __launch_bounds__(N) void k_obj_reduce_rows(Params p) {
   constexpr int K = 4;  // patterns whose loads are in flight together
   const int t = threadIdx.x;
   const int y = blockIdx.x, slot = blockIdx.y, s = p.s0 + slot;
-  const int x_off = t < N / 2 ? t : t - N;  // natural column offset of stored patch column t
+  const int ct = stored_col<N>(t);          // patch column at stored position t
+  const int x_off = ct < N / 2 ? ct : ct - N;  // its natural column offset
   const size_t slab = (size_t)p.wslots * N * N;  // stride between modes of one pattern
   for (int x = t; x < p.W; x += N) acc_row[x] = mk(0.0f, 0.0f);
   for (int b0 = 0; b0 < p.B; b0 += N) {
@@ -1349,6 +1377,10 @@ int qem_ptycho_forward_loss(const qem_ptycho_desc* desc, const void* obj, const 
     default: return launch_forward256(p, L, st);
   }
 }
+
+}  // namespace
+
+extern "C" {
 
 int qem_ptycho_adjoint(const qem_ptycho_desc* desc, const void* obj, const void* probe, const float* positions,
                        const void* propagators, const float* descan, const float* dloss, void* grad_obj,

@@ -131,6 +131,22 @@ __device__ __forceinline__ cf ld_stream(const cf* p) {
   return mk(x.x, x.y);
 }
 
+// Two consecutive complex64 values (16 bytes, 16-byte aligned) streamed out / in.
+typedef float f4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void st_stream2(cf* p, cf a, cf b) {
+  f4v x;
+  x.x = a.x;
+  x.y = a.y;
+  x.z = b.x;
+  x.w = b.y;
+  __builtin_nontemporal_store(x, (f4v*)p);
+}
+__device__ __forceinline__ void ld_stream2(const cf* p, cf& a, cf& b) {
+  const f4v x = __builtin_nontemporal_load((const f4v*)p);
+  a = mk(x.x, x.y);
+  b = mk(x.z, x.w);
+}
+
 // Object element at a byte offset from the slice base.
 __device__ __forceinline__ cf obj_at(const cf* O, unsigned byte_off) {
   return *(const cf*)((const char*)O + byte_off);

@@ -277,18 +277,11 @@ def _object_filters(out: torch.Tensor, obj_type: str, c: dict, mask, sampling) -
 
 
 def orthogonalize_probe(probe: torch.Tensor) -> torch.Tensor:
-    """Gram-Schmidt keeping each mode's norm, modes sorted by power (probe_models.py:484-525)."""
-    norms = torch.sqrt(torch.sum(probe.real.square() + probe.imag.square(), dim=(-2, -1), keepdim=True))
-    out = []
-    for i in range(probe.shape[0]):
-        p = probe[i]
-        for q in out:
-            p = p - torch.sum(q.conj() * p) * q
-        n = torch.sqrt(torch.sum(p.real.square() + p.imag.square())).clamp_min(1e-12)
-        out.append(p / n)
-    out = torch.stack(out) * norms.view(-1, 1, 1)
-    order = torch.argsort(torch.sum(torch.abs(out).square(), dim=(-2, -1)), descending=True)
-    return torch.complex(out.real[order], out.imag[order])
+    """Gram-Schmidt keeping each mode's norm, modes sorted by power (probe_models.py:484-525), on the
+    probe-orthogonalisation kernels (:func:`quantem_amd.constraints.orthogonalize_probe`)."""
+    from .constraints import orthogonalize_probe as ortho
+
+    return ortho(probe)
 
 
 def center_probe(probe: torch.Tensor) -> torch.Tensor:

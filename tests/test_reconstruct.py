@@ -84,16 +84,53 @@ def test_object_constraints():
     assert torch.allclose(pp.abs(), torch.ones(1, 4, 4), atol=1e-6)
 
 
-def test_orthogonalize_probe():
+def test_orthogonalize_probe_restatement():
+    """The torch restatement of the reference's Gram-Schmidt (the GPU kernels' checker): orthogonal
+    modes, sorted by power, norms kept."""
+    from oracle import torch_ref
+
     g = torch.Generator().manual_seed(0)
-    p = torch.randn((3, 16, 16), dtype=torch.complex64, generator=g)
-    q = rc.orthogonalize_probe(p)
+    p = torch.randn((3, 16, 16), dtype=torch.complex128, generator=g)
+    q = torch_ref.orthogonalize_probe_mgs(p)
     gram = torch.einsum("aij,bij->ab", q.conj(), q)
     off = gram - torch.diag(torch.diagonal(gram))
-    assert off.abs().max() < 1e-3 * gram.abs().max()
+    assert off.abs().max() < 1e-12 * gram.abs().max()
     pw = torch.sum(q.abs() ** 2, dim=(-2, -1))
     assert torch.all(pw[:-1] >= pw[1:])
-    assert torch.allclose(torch.sort(pw).values, torch.sort(torch.sum(p.abs() ** 2, dim=(-2, -1))).values, rtol=1e-4)
+    assert torch.allclose(torch.sort(pw).values, torch.sort(torch.sum(p.abs() ** 2, dim=(-2, -1))).values, rtol=1e-12)
+    # already orthogonal modes with distinct powers come back unchanged (up to the sort)
+    e = torch.zeros((2, 4, 4), dtype=torch.complex128)
+    e[0, 0, 0], e[1, 1, 1] = 1.0, 2.0
+    assert torch.equal(torch_ref.orthogonalize_probe_mgs(e), e.flip(0))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("m,n", [(2, 64), (4, 128), (8, 256), (3, 32), (16, 32)])
+def test_orthogonalize_probe_kernels_match_gram_schmidt(cuda, m, n):
+    """The probe-orthogonalisation kernels against the reference's Gram-Schmidt restated in torch
+    complex128 on the CPU: output (1e-5) and the gradient of a random linear functional (1e-4)."""
+    from oracle import torch_ref
+    from quantem_amd import constraints as qc
+    from quantem_amd import synth
+
+    base = synth.real_space_probe(n, 2.0 / n, 300e3, 20.0, 40.0)
+    w = np.linspace(1.0, 0.2, m)
+    p0 = synth.mixed_probe(base, w / w.sum(), 1e4)
+    rng = np.random.default_rng(m + n)
+    p0 = p0 + 0.3 * np.abs(p0).max() * (rng.normal(size=p0.shape) + 1j * rng.normal(size=p0.shape))  # not orthogonal
+    p0 = p0.astype(np.complex64)
+    gout = (rng.normal(size=p0.shape) + 1j * rng.normal(size=p0.shape)).astype(np.complex64)
+    ref_in = torch.from_numpy(p0.astype(np.complex128)).requires_grad_(True)
+    ref = torch_ref.orthogonalize_probe_mgs(ref_in)
+    ref.backward(torch.from_numpy(gout.astype(np.complex128)))
+    x = torch.from_numpy(p0).to(cuda).requires_grad_(True)
+    y = qc.orthogonalize_probe(x)
+    y.backward(torch.from_numpy(gout).to(cuda))
+    e_f = float((y.detach().cpu().to(torch.complex128) - ref.detach()).norm() / ref.detach().norm())
+    e_b = float((x.grad.cpu().to(torch.complex128) - ref_in.grad).norm() / ref_in.grad.norm())
+    print(f"orthogonalize m={m} n={n}: output {e_f:.2e}, gradient {e_b:.2e}", flush=True)
+    assert e_f < 1e-5
+    assert e_b < 1e-4
 
 
 def test_phase_nmse():
