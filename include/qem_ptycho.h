@@ -98,6 +98,26 @@ int qem_ptycho_forward_loss(const qem_ptycho_desc* desc, const void* obj, const 
                             const float* mask, const float* descan, float* loss, float* intensities,
                             void* workspace, size_t workspace_bytes, void* stream);
 
+/* Bytes of the static workspace of qem_ptycho_forward_loss_static (per GPU). */
+#define QEM_STATIC_WORKSPACE_BYTES ((size_t)64 << 20)
+
+/* qem_ptycho_forward_loss without a workspace argument, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  The no-gradient forward only (desc->flags must be 0: nothing is kept
+ * for an adjoint); its workspace is a file-scope device array of QEM_STATIC_WORKSPACE_BYTES bytes that
+ * each GPU holds from the first call on it until the process ends.  Requests whose workspace
+ * (qem_ptycho_workspace_bytes with flags 0) exceeds it return QEM_ERR_WORKSPACE: at most 111 patterns
+ * of 128 x 128 with 4 modes and 8 slices per call, 27 of 256 x 256 with 2 modes.  The call allocates nothing and
+ * calls no allocating routine; the workspace's address is looked up (hipGetSymbolAddress) on the first
+ * call on each device, for the device current at the call, which must be the device of `stream`.
+ * Every call on one GPU shares that array and nothing in it needs to start at zero (every call
+ * writes all it reads); calls that overlap on one GPU (other streams, other threads) race on it.
+ * Arguments, results and error codes otherwise as qem_ptycho_forward_loss. */
+int qem_ptycho_forward_loss_static(const qem_ptycho_desc* desc, const void* obj, const void* probe,
+                                   const float* positions, const void* propagators, const float* targets,
+                                   const float* mask, const float* descan, float* loss, float* intensities,
+                                   void* stream);
+
 /* Gradient of the batch loss (torch convention: grad = dL/dRe + i dL/dIm), scaled by *dloss
  * (device float, the upstream gradient).  Replaces torch autograd of the chain above
  * (invoked at ptychography.py:328).  Must follow qem_ptycho_forward_loss with the same

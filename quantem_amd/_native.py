@@ -25,6 +25,8 @@ QEM_FLAG_GRAD_POSITIONS = 1 << 1
 QEM_FLAG_GRAD_DESCAN = 1 << 2
 QEM_FLAG_ANALYTIC = 1 << 3
 
+QEM_STATIC_WORKSPACE_BYTES = 64 << 20  # workspace of qem_ptycho_forward_loss_static
+
 LOSS_TYPES = {
     "l2_amplitude": 0,
     "l1_amplitude": 1,
@@ -39,6 +41,7 @@ FFT_ROI = (32, 64, 128)  # sizes of the standalone fft2 entry point
 EXPORTED_SYMBOLS = (
     "qem_ptycho_workspace_bytes",
     "qem_ptycho_forward_loss",
+    "qem_ptycho_forward_loss_static",
     "qem_ptycho_adjoint",
     "qem_ptycho_projection_update",
     "qem_fft2_c64",
@@ -94,6 +97,8 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.qem_ptycho_workspace_bytes.restype = ctypes.c_int
     lib.qem_ptycho_forward_loss.argtypes = [dp] + [_vp] * 10 + [ctypes.c_size_t, _vp]
     lib.qem_ptycho_forward_loss.restype = ctypes.c_int
+    lib.qem_ptycho_forward_loss_static.argtypes = [dp] + [_vp] * 10
+    lib.qem_ptycho_forward_loss_static.restype = ctypes.c_int
     lib.qem_ptycho_adjoint.argtypes = [dp] + [_vp] * 11 + [ctypes.c_size_t, _vp]
     lib.qem_ptycho_adjoint.restype = ctypes.c_int
     lib.qem_ptycho_projection_update.argtypes = [dp] + [_vp] * 10 + [ctypes.c_size_t, _vp]

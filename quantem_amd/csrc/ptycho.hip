@@ -21,6 +21,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
+
 #include "common.h"
 #include "fft2.h"
 #include "ptycho_common.h"
@@ -1273,6 +1275,47 @@ int launch_projection_update256(const qem::Params& p, const Layout& L, float* ov
 
 }  // namespace
 
+// ---------------------------------------------------------------------------------------------
+// Workspace of the allocation-free forward (qem_ptycho_forward_loss_static): a file-scope device array,
+// one per GPU the library runs on, held until the process ends.  In host code `scratch` names a
+// host-side shadow of the array, not its device address: kernels get the address hipGetSymbolAddress
+// returns, looked up once per device (with that device current) on the first call there.
+static __device__ __align__(256) unsigned char scratch[QEM_STATIC_WORKSPACE_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's `scratch`.
+int scratch_address(void** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(QEM_ERR_LAUNCH, "hipGetDevice failed: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(QEM_ERR_UNSUPPORTED, "device %d: the static workspace supports devices 0..%d", dev, kMaxDevices - 1);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || ptr == nullptr)
+      return fail(QEM_ERR_LAUNCH, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return QEM_OK;
+}
+
+int check_forward_ptrs(const qem_ptycho_desc& d, const void* obj, const void* probe, const float* positions,
+                       const void* propagators, const float* targets, const float* mask, const float* descan,
+                       float* loss, float* intensities);
+int forward_loss_checked(const qem_ptycho_desc& d, const void* obj, const void* probe, const float* positions,
+                         const void* propagators, const float* targets, const float* mask, const float* descan,
+                         float* loss, float* intensities, void* workspace, size_t workspace_bytes, void* stream);
+
+}  // namespace
+
 extern "C" {
 
 int32_t qem_abi_version(void) { return 101; }
@@ -1341,7 +1384,39 @@ int qem_ptycho_forward_loss(const qem_ptycho_desc* desc, const void* obj, const 
                             float* loss, float* intensities, void* workspace, size_t workspace_bytes, void* stream) {
   qem::clear_error();
   if (int rc = check_desc(desc)) return rc;
-  const qem_ptycho_desc& d = *desc;
+  return forward_loss_checked(*desc, obj, probe, positions, propagators, targets, mask, descan, loss, intensities,
+                              workspace, workspace_bytes, stream);
+}
+
+int qem_ptycho_forward_loss_static(const qem_ptycho_desc* desc, const void* obj, const void* probe,
+                                   const float* positions, const void* propagators, const float* targets,
+                                   const float* mask, const float* descan, float* loss, float* intensities,
+                                   void* stream) {
+  qem::clear_error();
+  if (int rc = check_desc(desc)) return rc;
+  if (desc->flags != 0)
+    return fail(QEM_ERR_ARG, "the static-workspace forward runs the no-gradient forward only (flags must be 0)");
+  const size_t need = layout_of(*desc).total;
+  if (need > (size_t)QEM_STATIC_WORKSPACE_BYTES)
+    return fail(QEM_ERR_WORKSPACE,
+                "this call needs %zu bytes of workspace, more than the %zu of the static workspace; split the batch or "
+                "use qem_ptycho_forward_loss with a workspace",
+                need, (size_t)QEM_STATIC_WORKSPACE_BYTES);
+  if (int rc = check_forward_ptrs(*desc, obj, probe, positions, propagators, targets, mask, descan, loss, intensities))
+    return rc;
+  void* ws = nullptr;
+  if (int rc = scratch_address(&ws)) return rc;
+  return forward_loss_checked(*desc, obj, probe, positions, propagators, targets, mask, descan, loss, intensities, ws,
+                              (size_t)QEM_STATIC_WORKSPACE_BYTES, stream);
+}
+
+}  // extern "C"
+
+namespace {
+
+int check_forward_ptrs(const qem_ptycho_desc& d, const void* obj, const void* probe, const float* positions,
+                       const void* propagators, const float* targets, const float* mask, const float* descan,
+                       float* loss, float* intensities) {
   if (int rc = check_ptr(obj, "obj", true)) return rc;
   if (int rc = check_ptr(probe, "probe", true)) return rc;
   if (int rc = check_ptr(positions, "positions", false)) return rc;
@@ -1355,6 +1430,14 @@ int qem_ptycho_forward_loss(const qem_ptycho_desc* desc, const void* obj, const 
   if (int rc = check_ptr(loss, "loss", false)) return rc;
   if (intensities != nullptr)
     if (int rc = check_ptr(intensities, "intensities", false)) return rc;
+  return QEM_OK;
+}
+
+int forward_loss_checked(const qem_ptycho_desc& d, const void* obj, const void* probe, const float* positions,
+                         const void* propagators, const float* targets, const float* mask, const float* descan,
+                         float* loss, float* intensities, void* workspace, size_t workspace_bytes, void* stream) {
+  if (int rc = check_forward_ptrs(d, obj, probe, positions, propagators, targets, mask, descan, loss, intensities))
+    return rc;
   if (int rc = check_ptr(workspace, "workspace", true)) return rc;
   const Layout L = layout_of(d);
   if (workspace_bytes < L.total)
