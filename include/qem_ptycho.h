@@ -58,7 +58,10 @@ enum {
   QEM_FLAG_STORE_WAVES = 1u << 0, /* keep the per-slice incident waves for qem_ptycho_adjoint */
   QEM_FLAG_GRAD_POSITIONS = 1u << 1, /* adjoint also writes d loss / d positions */
   QEM_FLAG_GRAD_DESCAN = 1u << 2,    /* adjoint also writes d loss / d descan */
-  QEM_FLAG_ANALYTIC = 1u << 3        /* set by qem_ptycho_projection_update (internal) */
+  QEM_FLAG_ANALYTIC = 1u << 3,       /* set by qem_ptycho_projection_update (internal) */
+  QEM_FLAG_GRAD_PROPAGATORS = 1u << 4 /* with QEM_FLAG_STORE_WAVES and segment_slices 0: keep what
+                                         qem_ptycho_propagator_gradient needs (one more (S-1)-slice
+                                         spectrum per pattern and mode in the workspace) */
 };
 
 typedef struct qem_ptycho_desc {
@@ -133,6 +136,17 @@ int qem_ptycho_adjoint(const qem_ptycho_desc* desc, const void* obj, const void*
                        const float* dloss, void* grad_obj, void* grad_probe, float* grad_positions,
                        float* grad_descan, void* workspace, size_t workspace_bytes, void* stream);
 
+/* Gradient of the batch loss with respect to the propagators (torch convention), scaled by *dloss as
+ * in qem_ptycho_adjoint, after a forward_loss / adjoint pair run with QEM_FLAG_STORE_WAVES |
+ * QEM_FLAG_GRAD_PROPAGATORS (segment_slices 0) on the same descriptor, inputs and workspace:
+ *   grad_propagators (S-1, N, N) complex64 = sum over patterns and modes of
+ *                    conj(fft2(x_s)) fft2(dL/dpsi_{s+1}) / N^2,  x_s = O_s psi_s the transmitted wave
+ * summed in a fixed order.  Replaces torch autograd into the propagator tensor, which the reference
+ * recomputes from the probe tilt in every batch when learn_probe_tilt is set
+ * (ptychography_base.py:898-899, probe_models.py:376-427); the caller chains it into the tilt. */
+int qem_ptycho_propagator_gradient(const qem_ptycho_desc* desc, void* grad_propagators, void* workspace,
+                                   size_t workspace_bytes, void* stream);
+
 /* The reference's analytical update (autograd=False, ptychography.py:339-372) for one batch, after a
  * qem_ptycho_forward_loss call with QEM_FLAG_STORE_WAVES on the same descriptor, inputs and workspace:
  *   Fourier projection of every mode's exit wave onto the measured amplitudes `targets`
@@ -165,7 +179,7 @@ int qem_profile_end(float* fwd_ms, int32_t* n_fwd, float* adj_ms, int32_t* n_adj
 /* Thread-local description of the last error (empty string if none). */
 const char* qem_last_error(void);
 
-/* ABI version (major * 100 + minor): 101. */
+/* ABI version (major * 100 + minor): 102. */
 int32_t qem_abi_version(void);
 
 #ifdef __cplusplus

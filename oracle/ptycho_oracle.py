@@ -16,6 +16,7 @@ in float64/complex128 numpy.  How each one is pinned (independent checks in test
   batch_loss             hand-evaluated tiny cases; zero at targets == sqrt(I)
   adjoint                complex128 central finite differences; independent torch-autograd
                          restatement (oracle/torch_ref.py); dot-product tests per linear stage
+                         (the propagator gradient: the same finite differences and autograd)
 """
 
 from __future__ import annotations
@@ -178,6 +179,7 @@ def adjoint(obj, probe, positions, propagators, targets, mask, num_gpts, mean_in
                                   2 * np.pi * np.sum(f[None, None, None, :] * w.imag, axis=(0, 2, 3))], axis=-1)
         gx = gx * np.conj(ramp)[None]
     g_obj = np.zeros_like(obj)
+    g_prop = np.zeros((max(S - 1, 0), n, n), dtype=np.complex128)
     rows, cols = fw["rows"], fw["cols"]
     for s in range(S - 1, -1, -1):
         psi = fw["psis"][s]
@@ -186,8 +188,13 @@ def adjoint(obj, probe, positions, propagators, targets, mask, num_gpts, mean_in
             np.add.at(g_obj[s], (rows[b][:, None], cols[b][None, :]), gpatch[b])
         gpsi = np.conj(fw["patches"][s])[None] * gx
         if s > 0:
-            # adjoint of ifft2(fft2(.) * P) is ifft2(fft2(.) * conj(P))
-            gx = np.fft.ifft2(np.fft.fft2(gpsi) * np.conj(props[s - 1]))
+            # psi_s = ifft2(fft2(x_{s-1}) * P): the adjoint of ifft2(fft2(.) * P) is ifft2(fft2(.) * conj(P));
+            # d psi_s = ifft2(X dP) gives dL/dP = conj(X) fft2(g_psi) / n^2, X = fft2(x_{s-1}) (summed
+            # over modes and patterns)
+            G = np.fft.fft2(gpsi)
+            X = np.fft.fft2(fw["patches"][s - 1][None] * fw["psis"][s - 1])
+            g_prop[s - 1] = np.sum(np.conj(X) * G, axis=(0, 1)) / (n * n)
+            gx = np.fft.ifft2(G * np.conj(props[s - 1]))
     # probe: psi0 = ifft2(fft2(P) * R) => dP = sum_b ifft2(fft2(gpsi0) * conj(R))
     ramp = translation_ramp(fw["frac"], n)
     G = np.fft.fft2(gpsi)  # (M, B, n, n)
@@ -201,6 +208,7 @@ def adjoint(obj, probe, positions, propagators, targets, mask, num_gpts, mean_in
         np.real(np.sum(Z * (-2j * np.pi * k[None, None, None, :]), axis=(0, 2, 3))),
     ], axis=-1) / (n * n)
     out["obj"] = g_obj
+    out["propagators"] = g_prop
     out["loss"] = dloss * 0 + batch_loss(fw["I"], targets, mask, num_gpts, mean_intensity, loss_type)
     out["I"] = fw["I"]
     return out

@@ -100,7 +100,8 @@ def loss_and_grads(obj, probe, positions, propagators, targets, mask, num_gpts, 
     o = torch.tensor(np.asarray(obj), dtype=dtype, requires_grad=True)
     p = torch.tensor(np.asarray(probe), dtype=dtype, requires_grad=True)
     pos = torch.tensor(np.asarray(positions), dtype=rdt, requires_grad=grad_positions)
-    props = None if propagators is None or len(propagators) == 0 else torch.tensor(np.asarray(propagators), dtype=dtype)
+    props = None if propagators is None or len(propagators) == 0 else \
+        torch.tensor(np.asarray(propagators), dtype=dtype, requires_grad=True)
     t = torch.tensor(np.asarray(targets), dtype=rdt)
     m = None if mask is None else torch.tensor(np.asarray(mask), dtype=rdt)
     d = None if descan is None else torch.tensor(np.asarray(descan), dtype=rdt, requires_grad=True)
@@ -111,6 +112,8 @@ def loss_and_grads(obj, probe, positions, propagators, targets, mask, num_gpts, 
         out["positions"] = pos.grad.numpy()
     if d is not None:
         out["descan"] = d.grad.numpy()
+    if props is not None:
+        out["propagators"] = props.grad.numpy()
     return out
 
 

@@ -24,6 +24,7 @@ QEM_FLAG_STORE_WAVES = 1 << 0
 QEM_FLAG_GRAD_POSITIONS = 1 << 1
 QEM_FLAG_GRAD_DESCAN = 1 << 2
 QEM_FLAG_ANALYTIC = 1 << 3
+QEM_FLAG_GRAD_PROPAGATORS = 1 << 4
 
 QEM_STATIC_WORKSPACE_BYTES = 64 << 20  # workspace of qem_ptycho_forward_loss_static
 
@@ -44,6 +45,7 @@ EXPORTED_SYMBOLS = (
     "qem_ptycho_forward_loss_static",
     "qem_ptycho_adjoint",
     "qem_ptycho_projection_update",
+    "qem_ptycho_propagator_gradient",
     "qem_fft2_c64",
     "qem_last_error",
     "qem_abi_version",
@@ -103,6 +105,8 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.qem_ptycho_adjoint.restype = ctypes.c_int
     lib.qem_ptycho_projection_update.argtypes = [dp] + [_vp] * 10 + [ctypes.c_size_t, _vp]
     lib.qem_ptycho_projection_update.restype = ctypes.c_int
+    lib.qem_ptycho_propagator_gradient.argtypes = [dp, _vp, _vp, ctypes.c_size_t, _vp]
+    lib.qem_ptycho_propagator_gradient.restype = ctypes.c_int
     lib.qem_fft2_c64.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp, _vp, _vp]
     lib.qem_fft2_c64.restype = ctypes.c_int
     lib.qem_last_error.argtypes = []

@@ -98,7 +98,8 @@ struct ObjOffsets {
 // ALL: every incident wave is kept (FWD_STORE_ALL, FWD_RECOMPUTE), so the stores are not under a
 // branch; behind one, the wait-count analysis merged the two paths and made the transmission wait
 // for the stores ahead of it.
-template <int N, bool DESCAN, bool ALL>
+// PG: also keep fft2 of every transmitted wave that is propagated (p.pspec), for the propagator gradient.
+template <int N, bool DESCAN, bool ALL, bool PG = false>
 __global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
   __shared__ cf lds[Fft2<N>::LDS_TOTAL];
   using F = Fft2<N>;
@@ -177,6 +178,7 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
       // propagation: the propagator values of each pass-3 register group are loaded before that
       // group's last transform stage, so that their L2 latency hides behind it
       const cf* pr = p.prop + (size_t)s * N * N;
+      cf* xs = PG ? p.pspec + ((size_t)item * (p.S - 1) + s) * N * N : nullptr;
       cf q[F::RR3];
       f.fwd_groups(
           v, lds,
@@ -188,6 +190,7 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
 #pragma unroll
             for (int i = 0; i < F::RR3; ++i) {
               const int e = h.value * F::RR3 + i;
+              if constexpr (PG) st_stream(xs + f.p3_off(e), v[e]);
               v[e] = cscale(cmul(v[e], q[i]), inv_n2);
             }
           });
@@ -290,7 +293,8 @@ __global__ __launch_bounds__(256) void k_loss_finalize(Params p, int nparts) {
 //   g_psi = conj(O_s) g_x;  g_x = ifft2(fft2(g_psi) conj(prop_{s-1}))   (s > 0)
 // At s = 0: G = fft2(g_psi_0) and the probe-gradient spectrum conj(ramp_b) G is stored over X;
 // the position gradient reduces Im(conj(G) P_m ramp_b) against the frequencies.
-template <int N, bool DESCAN>
+// PG: also the propagator-gradient contributions (over p.pspec, see Params).
+template <int N, bool DESCAN, bool PG = false>
 __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
   __shared__ cf lds[Fft2<N>::LDS_TOTAL];
   __shared__ double red[Fft2<N>::T / 64];
@@ -424,17 +428,25 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
       // g_x <- ifft2(fft2(g_psi) conj(prop_{s-1})), the adjoint of the forward propagation, with the
       // propagator loads of each pass-3 register group issued before that group's last stage
       const cf* __restrict__ pr = p.prop + (size_t)(s - 1) * N * N;
+      cf* xs = PG ? p.pspec + ((size_t)item * (p.S - 1) + (s - 1)) * N * N : nullptr;
       cf q[F::RR3];
+      cf xq[PG ? F::RR3 : 1];
       f.fwd_groups(
           v, lds,
           [&](auto h) {
 #pragma unroll
             for (int i = 0; i < F::RR3; ++i) q[i] = pr[f.p3_off(h.value * F::RR3 + i)];
+            if constexpr (PG) {
+#pragma unroll
+              for (int i = 0; i < F::RR3; ++i) xq[i] = ld_stream(xs + f.p3_off(h.value * F::RR3 + i));
+            }
           },
           [&](auto h) {
 #pragma unroll
             for (int i = 0; i < F::RR3; ++i) {
               const int e = h.value * F::RR3 + i;
+              // dL/dP_{s-1} of this item: conj(fft2(x_{s-1})) fft2(g_psi_s) / N^2, over fft2(x_{s-1})
+              if constexpr (PG) st_stream(xs + f.p3_off(e), cscale(cconjmul(xq[i], v[e]), inv_n2));
               v[e] = cscale(cmulc(v[e], q[i]), inv_n2);
             }
           });
@@ -841,6 +853,35 @@ __global__ __launch_bounds__(256) void k_probe_reduce(Params p) {
   }
 }
 
+// grad_prop[s][k] = sum over the items (b, m) of their contributions in p.pspec, in a fixed order:
+// 64 frequencies of one slice per block, 4 lanes per frequency each summing every 4th item, combined in
+// lane order through LDS.
+template <int N>
+__global__ __launch_bounds__(256) void k_prop_reduce(Params p, cf* grad_prop) {
+  __shared__ cf part[4][64];
+  const int kk = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int idx = blockIdx.x * 64 + kk;  // over (S-1)*N*N (a multiple of 64)
+  const size_t slot = (size_t)(p.S - 1) * N * N;
+  const int items = p.B * p.M;
+  cf acc = mk(0.0f, 0.0f);
+#pragma unroll 8
+  for (int it = g; it < items; it += 4) {
+    const cf t = ld_stream(p.pspec + (size_t)it * slot + idx);
+    acc.x += t.x;
+    acc.y += t.y;
+  }
+  part[g][kk] = acc;
+  __syncthreads();
+  if (g == 0) {
+    cf t = part[0][kk];
+    for (int i = 1; i < 4; ++i) {
+      t.x += part[i][kk].x;
+      t.y += part[i][kk].y;
+    }
+    grad_prop[idx] = t;
+  }
+}
+
 // grad_probe[m] = ifft2(gsum[m]) (1/N^2 included).
 template <int N>
 __global__ __launch_bounds__(Fft2<N>::T) void k_probe_grad(Params p) {
@@ -954,7 +995,8 @@ void prof_mark(bool adjoint_kernel, int edge, hipStream_t st) {
 }
 
 struct Layout {
-  size_t psi, ckpt, scratch, X, gI, loss_part, pk, gsum, gpos, gdesc, ramps, total;
+  size_t psi, ckpt, scratch, X, gI, loss_part, pk, gsum, gpos, gdesc, ramps, pspec, total;
+  bool pgrad;  // propagator-gradient spectra kept (QEM_FLAG_GRAD_PROPAGATORS with stored waves)
   int nloss;
   int wslots;  // wave slots per item in psi
   int ck, nck; // checkpoint interval (0 = every incident wave is stored) and checkpoints per item
@@ -994,6 +1036,9 @@ Layout layout_of(const qem_ptycho_desc& d) {
   o += align_up(BM * 2 * 4);
   L.ramps = o;
   o += align_up((size_t)d.batch * 2 * d.roi * 8);
+  L.pgrad = store && (d.flags & QEM_FLAG_GRAD_PROPAGATORS) && S > 1;
+  L.pspec = o;
+  if (L.pgrad) o += align_up(BM * (size_t)(S - 1) * N2 * 8);
   L.total = o;
   return L;
 }
@@ -1027,6 +1072,8 @@ int check_desc(const qem_ptycho_desc* d) {
                 d->num_modes, d->roi);
   if (d->num_gpts < 1) return fail(QEM_ERR_ARG, "num_gpts must be >= 1 (got %d)", d->num_gpts);
   if (d->segment_slices < 0) return fail(QEM_ERR_ARG, "segment_slices must be >= 0 (got %d)", d->segment_slices);
+  if ((d->flags & QEM_FLAG_GRAD_PROPAGATORS) && d->segment_slices >= 2 && d->segment_slices < d->num_slices)
+    return fail(QEM_ERR_UNSUPPORTED, "the propagator gradient needs every incident wave stored (segment_slices 0)");
   if (d->loss_type < QEM_LOSS_L2_AMPLITUDE || d->loss_type > QEM_LOSS_POISSON)
     return fail(QEM_ERR_ARG, "unknown loss_type %d", d->loss_type);
   if (!(d->mean_intensity > 0.0f) || !isfinite(d->mean_intensity))
@@ -1073,6 +1120,7 @@ qem::Params make_params(const qem_ptycho_desc& d, const Layout& L, void* ws) {
   p.ramps = (qem::cf*)(w + L.ramps);
   p.gpos_part = (float*)(w + L.gpos);
   p.gdesc_part = (float*)(w + L.gdesc);
+  p.pspec = L.pgrad ? (qem::cf*)(w + L.pspec) : nullptr;
   return p;
 }
 
@@ -1086,7 +1134,11 @@ int launch_forward(const qem::Params& p, const Layout& L, hipStream_t st) {
   prof_mark(false, 0, st);
   // every incident wave kept: the store-every-slice instantiation (no branch around the stores)
   const bool all = p.fwd_mode == qem::FWD_STORE_ALL;
-  if (p.descan != nullptr && all)
+  if (p.pspec != nullptr && p.descan != nullptr)
+    hipLaunchKernelGGL((qem::k_forward<N, true, true, true>), dim3(p.B * p.M), dim3(F::T), 0, st, p);
+  else if (p.pspec != nullptr)
+    hipLaunchKernelGGL((qem::k_forward<N, false, true, true>), dim3(p.B * p.M), dim3(F::T), 0, st, p);
+  else if (p.descan != nullptr && all)
     hipLaunchKernelGGL((qem::k_forward<N, true, true>), dim3(p.B * p.M), dim3(F::T), 0, st, p);
   else if (p.descan != nullptr)
     hipLaunchKernelGGL((qem::k_forward<N, true, false>), dim3(p.B * p.M), dim3(F::T), 0, st, p);
@@ -1202,7 +1254,11 @@ int launch_adjoint(const qem::Params& p, hipStream_t st) {
       hipLaunchKernelGGL((qem::k_forward<N, false, true>), dim3(q.B * q.M), dim3(F::T), 0, st, q);
       if (int r = check_launch("forward (recompute)")) return r;
     }
-    if (q.descan != nullptr)
+    if (q.pspec != nullptr && q.descan != nullptr)
+      hipLaunchKernelGGL((qem::k_adjoint<N, true, true>), dim3(q.B * q.M), dim3(F::T), 0, st, q);
+    else if (q.pspec != nullptr)
+      hipLaunchKernelGGL((qem::k_adjoint<N, false, true>), dim3(q.B * q.M), dim3(F::T), 0, st, q);
+    else if (q.descan != nullptr)
       hipLaunchKernelGGL((qem::k_adjoint<N, true>), dim3(q.B * q.M), dim3(F::T), 0, st, q);
     else
       hipLaunchKernelGGL((qem::k_adjoint<N, false>), dim3(q.B * q.M), dim3(F::T), 0, st, q);
@@ -1318,7 +1374,7 @@ int forward_loss_checked(const qem_ptycho_desc& d, const void* obj, const void* 
 
 extern "C" {
 
-int32_t qem_abi_version(void) { return 101; }
+int32_t qem_abi_version(void) { return 102; }
 
 int qem_profile_begin(int32_t max_launches) {
   qem::clear_error();
@@ -1516,6 +1572,33 @@ int qem_ptycho_adjoint(const qem_ptycho_desc* desc, const void* obj, const void*
     case 128: return launch_adjoint<128>(p, st);
     default: return launch_adjoint256(p, st);
   }
+}
+
+int qem_ptycho_propagator_gradient(const qem_ptycho_desc* desc, void* grad_propagators, void* workspace,
+                                   size_t workspace_bytes, void* stream) {
+  qem::clear_error();
+  if (int rc = check_desc(desc)) return rc;
+  const qem_ptycho_desc& d = *desc;
+  const Layout L = layout_of(d);
+  if (d.num_slices < 2) return fail(QEM_ERR_ARG, "a single slice has no propagators");
+  if (!L.pgrad)
+    return fail(QEM_ERR_ARG, "the propagator gradient needs QEM_FLAG_STORE_WAVES | QEM_FLAG_GRAD_PROPAGATORS");
+  if (int rc = check_ptr(grad_propagators, "grad_propagators", true)) return rc;
+  if (int rc = check_ptr(workspace, "workspace", true)) return rc;
+  if (workspace_bytes < L.total)
+    return fail(QEM_ERR_WORKSPACE, "workspace has %zu bytes, needs %zu", workspace_bytes, L.total);
+  const qem::Params p = make_params(d, L, workspace);
+  const int N = d.roi;
+  const int blocks = (int)((size_t)(d.num_slices - 1) * N * N / 64);
+  hipStream_t st = (hipStream_t)stream;
+  qem::cf* g = (qem::cf*)grad_propagators;
+  switch (N) {
+    case 32: hipLaunchKernelGGL(qem::k_prop_reduce<32>, dim3(blocks), dim3(256), 0, st, p, g); break;
+    case 64: hipLaunchKernelGGL(qem::k_prop_reduce<64>, dim3(blocks), dim3(256), 0, st, p, g); break;
+    case 128: hipLaunchKernelGGL(qem::k_prop_reduce<128>, dim3(blocks), dim3(256), 0, st, p, g); break;
+    default: hipLaunchKernelGGL(qem::k_prop_reduce<256>, dim3(blocks), dim3(256), 0, st, p, g); break;
+  }
+  return check_launch("propagator_gradient");
 }
 
 int qem_ptycho_projection_update(const qem_ptycho_desc* desc, const void* obj, const void* probe,

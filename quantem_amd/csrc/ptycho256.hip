@@ -25,10 +25,16 @@ namespace n256 {
 
 constexpr int N = 256;
 constexpr int N2 = N * N;
-constexpr int T = 256;      // threads per workgroup (about 160 VGPRs: three workgroups per CU)
-constexpr int CH = T / 16;  // rows (or columns) per chunk: 16 lanes per row
-constexpr int VB = 272;     // LDS block per 16-lane vector: 16 x 17 (rows) or 16 x 16 + 16 (columns)
-static_assert(T == N, "one thread per frequency when tabulating the descan column factors");
+// Threads per workgroup: 512 (two waves per SIMD, up to 256 VGPRs each, 32 rows or columns per chunk)
+// measured 20 % faster in the forward and 6 % in the adjoint than 256 threads with the next chunk's
+// values carried in registers (EARLY_NEXT); 1024 threads (128 VGPRs) spill and ran slower.
+constexpr int T = 512;
+constexpr int CH = T / 16;   // rows (or columns) per chunk: 16 lanes per row
+constexpr int VB = 272;      // LDS block per 16-lane vector: 16 x 17 (rows) or 16 x 16 + 16 (columns)
+static_assert(T % 256 == 0 && T <= 1024, "whole 16-column groups in the column passes");
+// Carrying the next chunk's values in registers (EARLY_NEXT) pays with one wave per SIMD; with two the
+// other wave hides the load latency and the registers are needed for the chunk itself.
+constexpr bool kCarry = T <= 256;
 
 struct Smem {
   cf ex[CH * VB];  // exchange of the 16 x 16 decomposition
@@ -40,9 +46,9 @@ struct Smem {
 // The multislice kernels run one workgroup per CU: with 256 items resident the row-transformed waves
 // (256 x 512 KiB) stay in the Infinity Cache across the two passes of every slice, which measured
 // faster than two or three items per CU streaming twice that through HBM.  An LDS footprint above
-// half a CU's 160 KiB enforces it.
+// half a CU's 160 KiB enforces it (the exchange buffer of 32 vectors is 68 KiB; the rest is padding).
 struct SmemOne : Smem {
-  char occupancy_pad[82 * 1024 - sizeof(Smem)];
+  char occupancy_pad[sizeof(Smem) > 82 * 1024 ? 1 : 82 * 1024 - sizeof(Smem)];
 };
 static_assert(sizeof(SmemOne) > 80 * 1024, "one workgroup per CU");
 
@@ -103,7 +109,7 @@ struct NoPre {
 template <bool PRE_INV, bool POST_FWD, bool EARLY_NEXT = false, typename Op, typename Pre = NoPre>
 __device__ __forceinline__ void row_pass(const cf* src, cf* dst, Smem& sm, Op&& op, Pre&& pre = Pre()) {
   const int j = row_j();
-  if constexpr (EARLY_NEXT) {
+  if constexpr (EARLY_NEXT && kCarry) {
     cf v[16];
 #pragma unroll
     for (int e = 0; e < 16; ++e) v[e] = src[row_vec() * N + j + 16 * e];
@@ -132,6 +138,7 @@ __device__ __forceinline__ void row_pass(const cf* src, cf* dst, Smem& sm, Op&& 
       cf v[16];
 #pragma unroll
       for (int e = 0; e < 16; ++e) v[e] = src[r * N + j + 16 * e];
+      pre(r);  // the chunk's own values for op, loaded before its inverse row DFT
       if constexpr (PRE_INV) fft256<true, false>(v, sm);
       op(r, v);
       if constexpr (POST_FWD) fft256<false, false>(v, sm);
@@ -149,7 +156,7 @@ __device__ __forceinline__ void row_pass(const cf* src, cf* dst, Smem& sm, Op&& 
 template <bool EARLY_NEXT = false, typename Op>
 __device__ __forceinline__ void col_pass(const cf* src, cf* dst, Smem& sm, Op&& op) {
   const int j = col_j();
-  if constexpr (EARLY_NEXT) {
+  if constexpr (EARLY_NEXT && kCarry) {
     cf v[16];
 #pragma unroll
     for (int e = 0; e < 16; ++e) v[e] = src[(j + 16 * e) * N + col_vec()];
@@ -229,8 +236,8 @@ __global__ __launch_bounds__(T) void k_spectrum256(Params p) {
 }
 
 // Forward multislice of one (pattern, mode); the modes (p.fwd_mode) are those of k_forward<N>.
-template <bool DESCAN>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_forward256(Params p) {
+template <bool DESCAN, bool PG = false>
+__global__ __launch_bounds__(T) void k_forward256(Params p) {
   __shared__ SmemOne sm;
   init_tw(sm);
   const int item = xcd_item(blockIdx.x, gridDim.x);
@@ -328,9 +335,14 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(1, 1))) void 
     }
     if (s + 1 < p.s1) {
       const cf* pr = p.prop + (size_t)s * N2;
+      cf* xs = PG ? p.pspec + ((size_t)item * (p.S - 1) + s) * N2 : nullptr;
       col_pass(Wb, Wb, sm, [&](int kc, cf(&v)[16]) {
         fft256<false, true>(v, sm);
         const int jj = col_j();
+        if constexpr (PG) {  // fft2 of the transmitted wave, for the propagator gradient
+#pragma unroll
+          for (int e = 0; e < 16; ++e) st_stream(xs + (jj + 16 * e) * N + kc, v[e]);
+        }
 #pragma unroll
         for (int e = 0; e < 16; ++e) v[e] = cscale(cmul(v[e], pr[(jj + 16 * e) * N + kc]), inv_n2);
         fft256<true, true>(v, sm);
@@ -350,8 +362,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(1, 1))) void 
 // starts from the far field; a lower one from the row-transformed g that the segment above left in X.
 // The bottom segment ends with G = fft2(g_psi_0): probe-gradient spectrum conj(ramp) G over X and the
 // position-gradient partial sums; any other segment leaves its row-transformed g in X.
-template <bool DESCAN>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_adjoint256(Params p) {
+template <bool DESCAN, bool PG = false>
+__global__ __launch_bounds__(T) void k_adjoint256(Params p) {
   __shared__ SmemOne sm;
   init_tw(sm);
   const int item = xcd_item(blockIdx.x, gridDim.x);
@@ -395,7 +407,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(1, 1))) void 
     const cf* O = p.obj + (size_t)(p.S - 1) * HW;
     const cf* A = p.psi + ((size_t)item * p.wslots + (p.S - 1 - p.s0)) * N2;
     const float ddr = p.descan[2 * b];
-    sm.dcol[threadIdx.x] = expm2pi(fftfreq1<N>(threadIdx.x) * p.descan[2 * b + 1]);  // T == N
+    if (threadIdx.x < N) sm.dcol[threadIdx.x] = expm2pi(fftfreq1<N>(threadIdx.x) * p.descan[2 * b + 1]);
     __syncthreads();
     row_pass<true, true>(Wb, Wb, sm, [&](int r, cf(&v)[16]) {
       const unsigned orow = obj_row_off(r0, r, p.H, p.W);
@@ -447,12 +459,22 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(1, 1))) void 
       const cf* pr = p.prop + (size_t)(s - 1) * N2;
       // (the propagator column is loaded before the column DFT, which hides its latency: measured
       // faster here, slower in the forward's column pass)
+      cf* xs = PG ? p.pspec + ((size_t)item * (p.S - 1) + (s - 1)) * N2 : nullptr;
       col_pass<true>(Wb, s == p.s0 ? Xs : Wb, sm, [&](int kc, cf(&v)[16]) {
         const int jj = col_j();
         cf q[16];
+        cf xq[PG ? 16 : 1];
 #pragma unroll
         for (int e = 0; e < 16; ++e) q[e] = pr[(jj + 16 * e) * N + kc];
+        if constexpr (PG) {
+#pragma unroll
+          for (int e = 0; e < 16; ++e) xq[e] = ld_stream(xs + (jj + 16 * e) * N + kc);
+        }
         fft256<false, true>(v, sm);
+        if constexpr (PG) {  // this item's dL/dP_{s-1}: conj(fft2(x_{s-1})) fft2(g_psi_s) / N^2
+#pragma unroll
+          for (int e = 0; e < 16; ++e) st_stream(xs + (jj + 16 * e) * N + kc, cscale(cconjmul(xq[e], v[e]), inv_n2));
+        }
 #pragma unroll
         for (int e = 0; e < 16; ++e) v[e] = cscale(cmulc(v[e], q[e]), inv_n2);
         fft256<true, true>(v, sm);
@@ -582,7 +604,11 @@ int launch_chain256_forward(const Params& p, hipStream_t st) {
 }
 
 int launch_forward256_kernel(const Params& p, hipStream_t st) {
-  if (p.descan != nullptr && p.fwd_mode != FWD_RECOMPUTE)
+  if (p.pspec != nullptr && p.descan != nullptr)
+    hipLaunchKernelGGL((n256::k_forward256<true, true>), dim3(p.B * p.M), dim3(n256::T), 0, st, p);
+  else if (p.pspec != nullptr)
+    hipLaunchKernelGGL((n256::k_forward256<false, true>), dim3(p.B * p.M), dim3(n256::T), 0, st, p);
+  else if (p.descan != nullptr && p.fwd_mode != FWD_RECOMPUTE)
     hipLaunchKernelGGL((n256::k_forward256<true>), dim3(p.B * p.M), dim3(n256::T), 0, st, p);
   else
     hipLaunchKernelGGL((n256::k_forward256<false>), dim3(p.B * p.M), dim3(n256::T), 0, st, p);
@@ -590,7 +616,11 @@ int launch_forward256_kernel(const Params& p, hipStream_t st) {
 }
 
 int launch_chain256_adjoint(const Params& p, hipStream_t st) {
-  if (p.descan != nullptr)
+  if (p.pspec != nullptr && p.descan != nullptr)
+    hipLaunchKernelGGL((n256::k_adjoint256<true, true>), dim3(p.B * p.M), dim3(n256::T), 0, st, p);
+  else if (p.pspec != nullptr)
+    hipLaunchKernelGGL((n256::k_adjoint256<false, true>), dim3(p.B * p.M), dim3(n256::T), 0, st, p);
+  else if (p.descan != nullptr)
     hipLaunchKernelGGL((n256::k_adjoint256<true>), dim3(p.B * p.M), dim3(n256::T), 0, st, p);
   else
     hipLaunchKernelGGL((n256::k_adjoint256<false>), dim3(p.B * p.M), dim3(n256::T), 0, st, p);

@@ -49,6 +49,10 @@ struct Params {
   int ck, nck;
   int fwd_mode;  // FWD_* below
   cf* ckpt;      // (B*M*nck, N, N)
+  // propagator gradient (QEM_FLAG_GRAD_PROPAGATORS): the forward keeps fft2 of each slice's transmitted
+  // wave (P3 layout, slots s = 0..S-2 per item); the adjoint overwrites each with that item's
+  // contribution conj(X) fft2(g_psi) / N^2 to dL/dP_s, which a fixed-order reduce sums over the items
+  cf* pspec;     // (B*M*(S-1), N, N) or nullptr
 };
 
 // Layout signature of Params as compiled in the current translation unit: its size and every field's
@@ -61,7 +65,7 @@ struct Params {
   QF(S) QF(M) QF(B) QF(H) QF(W) QF(P) QF(loss_type) QF(mean_I) QF(flags) QF(obj) QF(probe) QF(pos) QF(prop)        \
   QF(targets) QF(mask) QF(descan) QF(loss_out) QF(intens_out) QF(dloss) QF(grad_obj) QF(grad_probe)           \
   QF(grad_pos) QF(grad_descan) QF(psi) QF(scratch) QF(X) QF(gI) QF(loss_part) QF(pk) QF(gsum) QF(ramps)          \
-  QF(gpos_part) QF(gdesc_part) QF(s0) QF(s1) QF(wslots) QF(ck) QF(nck) QF(fwd_mode) QF(ckpt)
+  QF(gpos_part) QF(gdesc_part) QF(s0) QF(s1) QF(wslots) QF(ck) QF(nck) QF(fwd_mode) QF(ckpt) QF(pspec)
 // internal linkage: each translation unit evaluates its own copy (an inline function would be merged)
 static constexpr unsigned long long params_signature_local() {
   unsigned long long h = sizeof(Params);

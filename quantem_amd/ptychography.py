@@ -34,6 +34,7 @@ __all__ = [
     "fft2",
     "ifft2",
     "workspace_bytes",
+    "fresnel_propagators",
 ]
 
 
@@ -101,9 +102,9 @@ class MultisliceLoss(torch.autograd.Function):
     """Loss of one batch; backward runs the hand-written adjoint.
 
     Inputs follow the layouts of the reference tensors (see ``include/qem_ptycho.h``).  Gradients
-    are returned for ``obj``, ``probe``, ``positions`` and ``descan`` (the last two only when they
-    require grad); propagators, targets and mask get ``None`` as in the reference, where they are
-    not optimised.  The backward pass consumes the workspace of its forward: differentiating the
+    are returned for ``obj``, ``probe``, ``positions``, ``descan`` and ``propagators`` (the last three
+    only when they require grad: propagators do when the caller builds them from a learned probe tilt,
+    ``ptychography_base.py:898-899``); targets and mask get ``None``.  The backward pass consumes the workspace of its forward: differentiating the
     same graph a second time (``retain_graph=True``) raises ``RuntimeError`` instead of returning
     wrong gradients, and the backward itself is not differentiable.
     """
@@ -144,7 +145,8 @@ class MultisliceLoss(torch.autograd.Function):
         desc_c = None if descan is None else _need(descan, "descan", torch.float32, (B, 2), device)
 
         # grad mode is always off inside Function.forward: the caller's mode arrives as grad_enabled
-        need_grad = bool(grad_enabled) and any(ctx.needs_input_grad[i] for i in (0, 1, 2, 6))
+        grad_prop = bool(grad_enabled) and prop_c is not None and ctx.needs_input_grad[3]
+        need_grad = bool(grad_enabled) and (any(ctx.needs_input_grad[i] for i in (0, 1, 2, 6)) or grad_prop)
         flags = 0
         if need_grad:
             flags |= nat.QEM_FLAG_STORE_WAVES
@@ -152,6 +154,8 @@ class MultisliceLoss(torch.autograd.Function):
                 flags |= nat.QEM_FLAG_GRAD_POSITIONS
             if desc_c is not None and ctx.needs_input_grad[6]:
                 flags |= nat.QEM_FLAG_GRAD_DESCAN
+            if grad_prop:
+                flags |= nat.QEM_FLAG_GRAD_PROPAGATORS
         d = _desc(S, M, B, N, H, W, num_gpts, loss_type, mean_intensity, flags, segment_slices)
         ws = torch.empty(workspace_bytes(d), dtype=torch.uint8, device=device)
         loss = torch.empty((), dtype=torch.float32, device=device)
@@ -193,9 +197,15 @@ class MultisliceLoss(torch.autograd.Function):
             ctypes.byref(d), _ptr(obj_c), _ptr(probe_c), _ptr(pos_c), _ptr(prop_c), _ptr(desc_c), _ptr(gl),
             _ptr(g_obj), _ptr(g_probe), _ptr(g_pos), _ptr(g_desc), _ptr(ws), ws.numel(), _stream(device))
         nat.check(rc, "qem_ptycho_adjoint")
+        g_prop = None
+        if d.flags & nat.QEM_FLAG_GRAD_PROPAGATORS:
+            g_prop = torch.empty_like(prop_c)
+            rc = lib.qem_ptycho_propagator_gradient(ctypes.byref(d), _ptr(g_prop), _ptr(ws), ws.numel(),
+                                                    _stream(device))
+            nat.check(rc, "qem_ptycho_propagator_gradient")
         return (g_obj if ctx.needs_input_grad[0] else None,
                 g_probe if ctx.needs_input_grad[1] else None,
-                g_pos, None, None, None, g_desc, None, None, None, None, None, None)
+                g_pos, g_prop, None, None, g_desc, None, None, None, None, None, None)
 
 
 def multislice_loss(
@@ -260,6 +270,34 @@ def forward_intensities(
         _, intens = multislice_loss(obj, probe, positions, propagators, tgt, None, descan, num_gpts=B,
                                     mean_intensity=1.0, return_intensities=True)
     return intens
+
+
+def fresnel_propagators(roi: int, sampling: float, energy: float, thicknesses, tilt_mrad: torch.Tensor,
+                        device=None) -> torch.Tensor:
+    """(S-1, N, N) complex64 Fresnel propagators with a probe tilt, differentiable in ``tilt_mrad``.
+
+    The reference's ``ProbeBase._compute_propagator_arrays`` (``probe_models.py:376-427``), op for op:
+    exp(-i pi lambda dz k^2) times, for a tilt component that is not exactly zero,
+    exp(-2 pi i dz tan(theta / 1e3) k) along that axis; a zero component contributes no factor, so (as
+    in the reference) it receives no gradient.  ``tilt_mrad`` is a (2,) float32 tensor (row, col) in mrad.
+    """
+    from .synth import electron_wavelength_angstrom
+
+    dev = tilt_mrad.device if device is None else torch.device(device)
+    kr = torch.fft.fftfreq(roi, sampling, device=dev)
+    kc = torch.fft.fftfreq(roi, sampling, device=dev)
+    k2 = (kr[:, None] ** 2 + kc[None] ** 2).to(torch.complex64)
+    wavelength = electron_wavelength_angstrom(energy)
+    dz = torch.tensor(list(thicknesses), device=dev, dtype=k2.dtype)
+    props = torch.exp((-1.0j * torch.pi * wavelength * dz[:, None, None]) * k2)
+    theta_r, theta_c = tilt_mrad[0], tilt_mrad[1]
+    if theta_r != 0:
+        kr_term = 1.0j * (-2 * torch.pi * dz[:, None, None] * torch.tan(theta_r / 1e3))
+        props = props * torch.exp(kr_term * kr[None, :, None])
+    if theta_c != 0:
+        kc_term = 1.0j * (-2 * torch.pi * dz[:, None, None] * torch.tan(theta_c / 1e3))
+        props = props * torch.exp(kc_term * kc[None, None, :])
+    return props
 
 
 def _fft2(x: torch.Tensor, inverse: bool) -> torch.Tensor:

@@ -304,3 +304,74 @@ def test_headline_size_batch_is_the_sum_of_its_parts(cuda):
     assert rel(full[4].cpu().numpy(), (f * torch.cat([pt[4] for pt in parts])).cpu().numpy()) < 1e-5
     sub = po.forward(case["obj"], case["probe"], case["positions"][:2], case["propagators"])["I"]
     assert rel(full[1][:2].cpu().numpy(), sub) < FWD_TOL
+
+
+@pytest.mark.parametrize("n,s,m,b,descan", [(32, 3, 2, 4, True), (64, 4, 2, 3, False), (128, 3, 2, 3, False),
+                                            (128, 8, 4, 2, False), (256, 3, 2, 2, True)])
+def test_propagator_gradient_matches_oracle(cuda, n, s, m, b, descan):
+    """dL/dpropagators (the gradient a learned probe tilt chains through) against the FP64 oracle; the
+    other gradients are unchanged by the extra output."""
+    print(f"propagator gradient n={n} s={s} m={m} b={b} descan={descan}", flush=True)
+    case = _case_with_targets(N=n, S=s, M=m, B=b, seed=900 + n + s, descan=descan)
+    dev = cuda
+    obj = _t(case["obj"], dev).requires_grad_(True)
+    probe = _t(case["probe"], dev).requires_grad_(True)
+    props = _t(case["propagators"], dev).requires_grad_(True)
+    loss = qp.multislice_loss(obj, probe, _t(case["positions"], dev), props, _t(case["targets"], dev),
+                              _t(case["mask"], dev), _t(case["descan"], dev), num_gpts=case["num_gpts"],
+                              mean_intensity=case["mean_intensity"])
+    loss.backward()
+    ref = _oracle(case)
+    e = rel(props.grad.cpu().numpy(), ref["propagators"])
+    print(f"  propagators {e:.2e}", flush=True)
+    assert e < GRAD_TOL
+    assert rel(obj.grad.cpu().numpy(), ref["obj"]) < GRAD_TOL
+    assert rel(probe.grad.cpu().numpy(), ref["probe"]) < GRAD_TOL
+    if s > 2:  # with recomputation segments the propagator gradient is refused on the host
+        with pytest.raises(ValueError):
+            qp.multislice_loss(obj, probe, _t(case["positions"], dev), props, _t(case["targets"], dev),
+                               _t(case["mask"], dev), None, num_gpts=case["num_gpts"],
+                               mean_intensity=case["mean_intensity"], segment_slices=2)
+
+
+def _tilt_props64(n, sampling, dz, tilt):
+    """float64 Fresnel propagators with a tilt (the formula of probe_models.py:404-425)."""
+    lam = synth.electron_wavelength_angstrom(300e3)
+    k = np.fft.fftfreq(n, sampling)
+    k2 = k[:, None] ** 2 + k[None, :] ** 2
+    out = []
+    for d in dz:
+        pr = np.exp(-1j * np.pi * lam * d * k2)
+        pr = pr * np.exp(-2j * np.pi * d * np.tan(tilt[0] / 1e3) * k)[:, None]
+        pr = pr * np.exp(-2j * np.pi * d * np.tan(tilt[1] / 1e3) * k)[None, :]
+        out.append(pr)
+    return np.stack(out)
+
+
+def test_probe_tilt_gradient_matches_finite_differences(cuda):
+    """The probe-tilt gradient through the kernels' propagator gradient and the torch propagator builder
+    against complex128 central finite differences of the oracle loss in the tilt (h = 1e-3 mrad)."""
+    n, S, samp = 64, 4, 0.1
+    dz = [8.0] * (S - 1)
+    tilt = np.array([6.0, -4.0])
+    case = make_case(N=n, S=S, M=2, B=4, seed=31, sampling=samp)
+    case["propagators"] = _tilt_props64(n, samp, dz, tilt).astype(np.complex64)
+    I = po.forward(case["obj"], case["probe"], case["positions"], case["propagators"])["I"]
+    case = attach_targets(case, I)
+    dev = cuda
+    th = torch.tensor(tilt, dtype=torch.float32, device=dev, requires_grad=True)
+    props = qp.fresnel_propagators(n, samp, 300e3, dz, th)
+    loss = qp.multislice_loss(_t(case["obj"], dev), _t(case["probe"], dev), _t(case["positions"], dev), props,
+                              _t(case["targets"], dev), _t(case["mask"], dev), num_gpts=case["num_gpts"],
+                              mean_intensity=case["mean_intensity"])
+    loss.backward()
+    got = th.grad.cpu().numpy()
+
+    def L(t):
+        I = po.forward(case["obj"], case["probe"], case["positions"], _tilt_props64(n, samp, dz, t))["I"]
+        return po.batch_loss(I, case["targets"], case["mask"], case["num_gpts"], case["mean_intensity"])
+
+    h = 1e-3
+    fd = np.array([(L(tilt + h * e) - L(tilt - h * e)) / (2 * h) for e in np.eye(2)])
+    print(f"tilt gradient: kernels {got}, finite differences {fd}", flush=True)
+    assert np.allclose(got, fd, rtol=2e-3, atol=1e-4 * np.abs(fd).max())

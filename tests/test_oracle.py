@@ -143,7 +143,7 @@ def test_adjoint_matches_torch_autograd(loss_type):
                                  case["targets"], case["mask"], case["num_gpts"], case["mean_intensity"],
                                  loss_type, descan=case["descan"])
     assert abs(a["loss"] - b["loss"]) <= 1e-12 * abs(b["loss"])
-    for k in ("obj", "probe", "positions", "descan"):
+    for k in ("obj", "probe", "positions", "descan", "propagators"):
         assert rel(a[k], b[k]) < 1e-10, k
 
 
@@ -182,6 +182,18 @@ def test_adjoint_matches_finite_differences():
             e[m, i, j] = h * unit
             fd = (L(p=probe + e) - L(p=probe - e)) / (2 * h)
             an = g["probe"][m, i, j].real if unit == 1.0 else g["probe"][m, i, j].imag
+            assert fd == pytest.approx(an, rel=1e-5, abs=1e-9)
+    def Lp(pp):
+        I = po.forward(obj, probe, pos, pp, desc)["I"]
+        return po.batch_loss(I, case["targets"], case["mask"], case["num_gpts"], case["mean_intensity"])
+
+    for _ in range(6):  # propagator entries (the gradient that learning a probe tilt chains through)
+        i, j = rng.integers(0, 16), rng.integers(0, 16)
+        for unit in (1.0, 1j):
+            e = np.zeros_like(props)
+            e[0, i, j] = h * unit
+            fd = (Lp(props + e) - Lp(props - e)) / (2 * h)
+            an = g["propagators"][0, i, j].real if unit == 1.0 else g["propagators"][0, i, j].imag
             assert fd == pytest.approx(an, rel=1e-5, abs=1e-9)
     for b in range(3):
         for k in range(2):

@@ -313,6 +313,38 @@ def test_optimizer_types_drive_the_loop(cuda, opt):
         rec.reconstruct(1, 32, optimizers={"positions": {"type": "sgd"}})
 
 
+@pytest.mark.gpu
+def test_probe_tilt_refinement(cuda):
+    """Data simulated with a probe tilt of (12, -8) mrad through 3 slices 10 A apart; with the object and
+    probe known, learning the tilt from (3, 3) mrad moves it towards the true value (the reference's
+    learn_probe_tilt: propagators rebuilt from the tilt every batch, probe_models.py:376-427)."""
+    from quantem_amd.ptychography import forward_intensities
+
+    spec = synth.ProblemSpec(name="tilt-test", num_slices=3, obj_shape=(128, 128), roi=64, num_modes=1,
+                             mode_weights=(1.0,), scan=(12, 12), scan_step=4.0, jitter=0.0, dose=1e6, n_atoms=120,
+                             batch=32, slice_thickness=10.0)
+    prob = synth.make_problem(spec, seed=2)
+    true_tilt = (12.0, -8.0)
+    props = synth.fresnel_propagators(64, spec.sampling, spec.energy, [10.0, 10.0], true_tilt)
+    I = forward_intensities(torch.from_numpy(prob["obj"]).to(cuda), torch.from_numpy(prob["probe"]).to(cuda),
+                            torch.from_numpy(prob["positions"]).to(cuda), torch.from_numpy(props).to(cuda))
+    amps, mean_I = synth.poisson_amplitudes(I.cpu().numpy(), np.random.default_rng(3))
+    rec = rc.Reconstruction(prob["positions"], amps, None, prob["probe"], (3,) + spec.obj_shape, mean_I,
+                            upsample_factor=synth.spec_upsample_factor(spec), obj_init=prob["obj"],
+                            probe_tilt=(3.0, 3.0),
+                            propagator_params={"sampling": spec.sampling, "energy": spec.energy,
+                                               "thicknesses": [10.0, 10.0]}, device=cuda)
+    hist = rec.reconstruct(15, 32, obj_lr=0.0, learn_probe=False, learn_probe_tilt=True, seed=0,
+                           optimizers={"probe": {"type": "adam", "lr": 1.0}})
+    err0 = float(np.hypot(3.0 - true_tilt[0], 3.0 - true_tilt[1]))
+    t = rec.tilt_history[-1]
+    err = float(np.hypot(t[0] - true_tilt[0], t[1] - true_tilt[1]))
+    print(f"probe tilt: (3, 3) -> ({t[0]:.2f}, {t[1]:.2f}) mrad, true {true_tilt}; loss {hist[0]:.4g} -> "
+          f"{hist[-1]:.4g}", flush=True)
+    assert hist[-1] < hist[0]
+    assert err < 0.3 * err0
+
+
 def test_soft_constraint_losses_known_answers():
     """TV / surface-zero losses of the reference's object and probe soft constraints on hand-sized arrays
     (object_models.py:341-451, probe_models.py:441-467)."""

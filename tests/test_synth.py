@@ -100,3 +100,28 @@ def test_upsample_factor_of_the_workloads(name, value):
     """upsample_factor = floor(fov / sampling) (rounded up to even) / gpts; at the 64 x 64 scan with a
     4 px step: fov = 63 * 0.4 A = 25.2 A -> 252 px -> 252 / 64 = 3.9375."""
     assert synth.spec_upsample_factor(synth.WORKLOADS[name]) == pytest.approx(value, abs=1e-12)
+
+
+@pytest.mark.parametrize("tilt", [(0.0, 0.0), (3.0, -2.0), (0.0, 5.0)])
+def test_torch_propagator_builder_matches_numpy(tilt):
+    """The differentiable propagator builder of the loop (torch, complex64) against the numpy one that
+    simulates the data (probe_models.py:376-427)."""
+    import torch
+
+    from quantem_amd import ptychography as qp
+
+    a = qp.fresnel_propagators(64, 0.1, 300e3, [2.0, 1.5], torch.tensor(tilt)).numpy()
+    b = synth.fresnel_propagators(64, 0.1, 300e3, [2.0, 1.5], tilt)
+    assert np.abs(a - b).max() < 1e-6
+
+
+def test_zero_tilt_component_gets_no_gradient():
+    """As in the reference, a tilt component that is exactly zero adds no factor and so no gradient."""
+    import torch
+
+    from quantem_amd import ptychography as qp
+
+    th = torch.tensor([0.0, 2.0], requires_grad=True)
+    p = qp.fresnel_propagators(32, 0.1, 300e3, [5.0], th)
+    (p.real.sum() + p.imag.sum()).backward()
+    assert th.grad[0] == 0 and th.grad[1] != 0
