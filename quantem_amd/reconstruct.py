@@ -676,6 +676,8 @@ def main(argv=None) -> int:
     import json
     import sys
 
+    import scipy.ndimage
+
     from . import synth
 
     ap = argparse.ArgumentParser(description=main.__doc__)
@@ -689,6 +691,11 @@ def main(argv=None) -> int:
     ap.add_argument("--val-ratio", type=float, default=0.0, help="fraction of positions held out for validation")
     ap.add_argument("--schedule", default=None, choices=["cyclic", "plateau", "exp", "linear"],
                     help="learning-rate schedule of the object and probe optimizers (reference defaults)")
+    ap.add_argument("--obj-type", default="complex", choices=["complex", "pure_phase", "potential"],
+                    help="the reference's object types (object_models.py:289-339)")
+    ap.add_argument("--tv-z", type=float, default=0.0, help="object soft constraint tv_weight_z")
+    ap.add_argument("--tv-xy", type=float, default=0.0, help="object soft constraint tv_weight_xy")
+    ap.add_argument("--obj-lr", type=float, default=5e-3, help="object learning rate (the reference's 5e-3)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default=None, help="write the summary JSON here")
     args = ap.parse_args(argv)
@@ -706,19 +713,34 @@ def main(argv=None) -> int:
     true_phase = prob["true_phase"].sum(0)
     crop = scan_window(spec)
     rec = Reconstruction(prob["positions"], amps, prob["propagators"], prob["probe"],
-                         (spec.num_slices,) + tuple(spec.obj_shape), mean_I,
+                         (spec.num_slices,) + tuple(spec.obj_shape), mean_I, obj_type=args.obj_type,
                          upsample_factor=synth.spec_upsample_factor(spec),
+                         object_constraints={"tv_weight_z": args.tv_z, "tv_weight_xy": args.tv_xy},
                          device=dev)
-    nmse_hist = []
+    nmse_hist, diag_hist = [], []
+    (r0, r1), (c0, c1) = crop
 
     def report(epoch, loss, r):
-        est = projected_phase(r.obj_constrained.detach()).cpu().numpy()
+        oc = r.obj_constrained.detach()
+        est = projected_phase(oc).cpu().numpy()
         nmse_hist.append(phase_nmse(est, true_phase, crop))
-        print(f"[reconstruct] epoch {epoch + 1}: loss {loss:.5g}, phase NMSE {nmse_hist[-1]:.4f}, "
-              f"{r.times[-1]:.1f} s", flush=True)
+        # the same error after a Gaussian low-pass (sigma 1.5 px, the atoms' width) of both phases:
+        # separates the drift of weakly constrained high spatial frequencies from that of the atoms
+        lp = lambda x: scipy.ndimage.gaussian_filter(x, 1.5, mode="wrap")  # noqa: E731
+        lowpass_nmse = phase_nmse(lp(est), lp(true_phase), crop)
+        # where the per-slice phases sit: values near +-pi wrap in the projected phase (a sum of angles)
+        ang = torch.angle(oc[:, r0:r1, c0:c1])
+        diag = {"lowpass_phase_nmse": lowpass_nmse, "max_abs_slice_phase": float(ang.abs().max()), "frac_abs_phase_gt_2_5": float((ang.abs() > 2.5).float().mean()),
+                "mean_amplitude": float(oc[:, r0:r1, c0:c1].abs().mean()),
+                "slice_phase_std": [round(float(x), 4) for x in ang.std(dim=(1, 2)).cpu()]}
+        diag_hist.append(diag)
+        print(f"[reconstruct] epoch {epoch + 1}: loss {loss:.5g}, phase NMSE {nmse_hist[-1]:.4f} "
+              f"(low-pass {lowpass_nmse:.4f}), max |slice phase| {diag['max_abs_slice_phase']:.3f}, |phase| > 2.5 rad {diag['frac_abs_phase_gt_2_5']:.2e}, "
+              f"mean amplitude {diag['mean_amplitude']:.4f}, {r.times[-1]:.1f} s", flush=True)
 
     torch.cuda.reset_peak_memory_stats(dev)
-    rec.reconstruct(args.epochs, args.batch or spec.batch, learn_probe=args.learn_probe, seed=args.seed,
+    rec.reconstruct(args.epochs, args.batch or spec.batch, obj_lr=args.obj_lr, learn_probe=args.learn_probe,
+                    seed=args.seed,
                     segment_slices=args.segment, dataset_lr=args.dataset_lr, val_ratio=args.val_ratio,
                     schedulers=({"object": {"type": args.schedule}, "probe": {"type": args.schedule}}
                                 if args.schedule else None),
@@ -730,7 +752,9 @@ def main(argv=None) -> int:
            "patterns_per_s": round(P * args.epochs / rec.times[-1], 1),
            "final_loss": rec.history[-1], "final_phase_nmse": nmse_hist[-1],
            "final_val_loss": rec.val_history[-1] if rec.val_history else None,
-           "phase_nmse_per_epoch": [round(x, 5) for x in nmse_hist],
+           "phase_nmse_per_epoch": [round(x, 5) for x in nmse_hist], "diagnostics_per_epoch": diag_hist,
+           "obj_type": args.obj_type, "tv_weight_z": args.tv_z, "tv_weight_xy": args.tv_xy, "obj_lr": args.obj_lr,
+           "true_slice_phase_std": [round(float(x), 4) for x in prob["true_phase"][:, r0:r1, c0:c1].std(axis=(1, 2))],
            "peak_hbm_bytes": int(torch.cuda.max_memory_allocated(dev)), "simulation_s": round(t_data, 2)}
     print(json.dumps(out), flush=True)
     if args.out:

@@ -208,6 +208,7 @@ class WarmStartSpec:
     sampling: float = 0.1
     semiangle_mrad: float = 20.0
     defocus: float = 100.0
+    mode_weights: tuple = (1.0,)
 
     @property
     def fov(self) -> int:
@@ -236,6 +237,14 @@ class WarmStartSpec:
         return int(pad)
 
 
+# Training pairs with the physics of the large workload (30 slices of 1.5 A, 256^2 patterns at 0.1 A/px,
+# 8 probe modes, 1e5 e-, the headline's atom density of 0.002 per px^2 and slice) on 384^2 objects
+# scanned 32 x 32: the network is fully convolutional over the scan grid, so it runs on the large
+# workload's 128 x 128 scan with 64 cells of padding (the 1024^2 object).
+LARGE_PAIRS = WarmStartSpec(num_slices=30, obj_size=384, roi=256, scan=32, density=(0.002, 0.002), dose=1e5,
+                            slice_thickness=1.5, mode_weights=(0.86,) + (0.02,) * 7)
+
+
 def atom_phase_fft(size: int, n_atoms: int, sigma: float, amp_range, rng: np.random.Generator) -> np.ndarray:
     """Periodic Gaussian-atom phase by splatting weighted deltas and convolving in Fourier space."""
     pos = rng.uniform(0, size, (n_atoms, 2))
@@ -261,7 +270,7 @@ class PairSimulator:
         self.spec = spec
         self.dev = torch.device(device)
         base = synth.real_space_probe(spec.roi, spec.sampling, spec.energy, spec.semiangle_mrad, spec.defocus)
-        self.probe_np = synth.mixed_probe(base, [1.0], spec.dose).astype(np.complex64)
+        self.probe_np = synth.mixed_probe(base, list(spec.mode_weights), spec.dose).astype(np.complex64)
         self.probe = torch.from_numpy(self.probe_np).to(self.dev)
         self.props_np = synth.fresnel_propagators(spec.roi, spec.sampling, spec.energy,
                                                   [spec.slice_thickness] * (spec.num_slices - 1))
@@ -296,12 +305,14 @@ class PairSimulator:
                 "mean_intensity": float(counts.sum(dim=(-2, -1)).mean())}
 
 
-def make_dataset(sim: PairSimulator, seeds) -> tuple:
+def make_dataset(sim: PairSimulator, seeds, log=None) -> tuple:
     feats, targets = [], []
-    for sd in seeds:
+    for i, sd in enumerate(seeds):
         d = sim.simulate(int(sd))
         feats.append(d["features"])
         targets.append(d["target"])
+        if log and (i + 1) % 100 == 0:
+            log(f"simulated {i + 1} pairs")
     return torch.stack(feats), torch.stack(targets)
 
 
@@ -399,8 +410,61 @@ def evaluate(net: WarmStartNet, norm: FeatureNorm, sim: PairSimulator, seed: int
             "ratio": t_warm / flat[-1][0] if flat[-1][0] > 0 else math.inf, "t_init": t_init}
 
 
+def evaluate_large(net: WarmStartNet, norm: FeatureNorm, sim: PairSimulator, epochs: int = 50, seed: int = 0,
+                   object_constraints: Optional[dict] = None, schedule: Optional[str] = None, log=None) -> dict:
+    """Flat start against the learned warm start on the large workload (30 x 1024^2, 8 modes, 256^2
+    patterns, 128 x 128 scan, batch 512): ``epochs`` of the loop mirror each, projected-phase NMSE over
+    the scanned window after every epoch; the warm start's clock includes features and inference."""
+    from .reconstruct import scan_window, simulate_amplitudes
+
+    spec = synth.LARGE
+    dev = sim.dev
+    prob = synth.make_problem(spec, seed=seed)
+    amps, mean_I = simulate_amplitudes(spec, prob, dev, seed)
+    true = prob["true_phase"].sum(0)
+    crop = scan_window(spec)
+    shape = (spec.num_slices,) + tuple(spec.obj_shape)
+    sched = {"object": {"type": schedule}} if schedule else None
+
+    def run(obj_init, t_offset, name):
+        rec = Reconstruction(prob["positions"], amps, prob["propagators"], prob["probe"], shape, mean_I,
+                             upsample_factor=synth.spec_upsample_factor(spec), obj_init=obj_init,
+                             object_constraints=object_constraints, device=dev)
+        curve = [(t_offset, phase_nmse(projected_phase(rec.obj_constrained.detach()).cpu().numpy(), true, crop))]
+
+        def cb(ep, loss, r):
+            est = projected_phase(r.obj_constrained.detach()).cpu().numpy()
+            curve.append((t_offset + r.times[-1], phase_nmse(est, true, crop)))
+            if log:
+                log(f"{name} epoch {ep + 1}: loss {loss:.5g}, phase NMSE {curve[-1][1]:.4f}, {curve[-1][0]:.1f} s")
+
+        rec.reconstruct(epochs, spec.batch, learn_probe=False, seed=seed, schedulers=sched, callback=cb)
+        return curve
+
+    torch.cuda.synchronize()
+    flat = run(None, 0.0, "flat")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    counts = amps ** 2
+    f = pattern_features(counts, sim.edges, amplitudes=False).view(spec.scan[0], spec.scan[1], 8).permute(2, 0, 1)
+    old_pad = net.pad
+    net.pad = int((spec.obj_shape[0] / spec.scan_step - spec.scan[0]) / 2)
+    phi = predict_phase(net, norm, f)
+    net.pad = old_pad
+    init = torch.exp(1j * (phi / spec.num_slices)).to(torch.complex64).expand(shape).contiguous()
+    torch.cuda.synchronize()
+    t_init = time.perf_counter() - t0
+    warm = run(init, t_init, "warm")
+    target = flat[-1][1]
+    t_warm = next((t for t, e in warm if e <= target), math.inf)
+    return {"flat": flat, "warm": warm, "target_nmse": target, "t_flat": flat[-1][0], "t_warm_to_target": t_warm,
+            "ratio": t_warm / flat[-1][0] if flat[-1][0] > 0 else math.inf, "t_init": t_init,
+            "patterns_per_s_flat": amps.shape[0] * epochs / flat[-1][0]}
+
+
 def main(argv=None) -> int:
-    """python -m quantem_amd.warmstart --train 400 --epochs 20 --eval 3"""
+    """python -m quantem_amd.warmstart --train 400 --epochs 20 --eval 3
+    python -m quantem_amd.warmstart --config large --train 600 --epochs 20 --rec-epochs 50"""
     import argparse
     import json
     import sys
@@ -413,16 +477,22 @@ def main(argv=None) -> int:
     ap.add_argument("--eval", type=int, default=3, help="held-out objects (seeds 100000+)")
     ap.add_argument("--rec-epochs", type=int, default=20)
     ap.add_argument("--out", type=str, default="")
+    ap.add_argument("--config", default="small", choices=["small", "large"],
+                    help="small: 4-slice 256^2 objects (evaluated on held-out objects of the same kind); large: "
+                         "pairs with the large workload's physics, evaluated on the 30 x 1024^2 workload itself")
+    ap.add_argument("--tv-xy", type=float, default=0.0, help="object tv_weight_xy of both large-workload runs")
+    ap.add_argument("--schedule", default=None, choices=["exp", "cyclic", "plateau", "linear"],
+                    help="object learning-rate schedule of both large-workload runs")
     args = ap.parse_args(argv)
     dev = torch.device("cuda:0")
 
     def log(m):
         print(f"[warmstart] {m}", file=sys.stderr, flush=True)
 
-    spec = WarmStartSpec()
+    spec = WarmStartSpec() if args.config == "small" else LARGE_PAIRS
     sim = PairSimulator(spec, dev)
     t0 = time.perf_counter()
-    feats, targets = make_dataset(sim, range(args.train))
+    feats, targets = make_dataset(sim, range(args.train), log=log)
     torch.cuda.synchronize()
     t_sim = time.perf_counter() - t0
     log(f"simulated {args.train} pairs in {t_sim:.1f} s")
@@ -434,6 +504,21 @@ def main(argv=None) -> int:
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     log(f"trained {args.epochs} epochs in {t_train:.1f} s")
+    if args.config == "large":
+        cons = {"tv_weight_xy": args.tv_xy} if args.tv_xy else None
+        r = evaluate_large(net, norm, sim, epochs=args.rec_epochs, object_constraints=cons, schedule=args.schedule,
+                           log=log)
+        log(f"large workload: flat final NMSE {r['target_nmse']:.4f} in {r['t_flat']:.1f} s; warm start initial "
+            f"NMSE {r['warm'][0][1]:.4f}, final {r['warm'][-1][1]:.4f}, reaches the flat final NMSE in "
+            f"{r['t_warm_to_target']:.1f} s (ratio {r['ratio']:.3f})")
+        summary = {"config": "large", "train_pairs": args.train, "train_epochs": args.epochs, "train_loss": hist,
+                   "t_simulate_s": t_sim, "t_train_s": t_train, "rec_epochs": args.rec_epochs, "tv_weight_xy": args.tv_xy,
+                   "schedule": args.schedule, **r}
+        if args.out:
+            with open(args.out, "w") as f:
+                json.dump(summary, f)
+        print(json.dumps({k: v for k, v in summary.items() if k not in ("flat", "warm", "train_loss")}), flush=True)
+        return 0
     results = []
     if args.eval > 0:
         # one untimed evaluation of an object outside the held-out seeds first, so that one-time costs
