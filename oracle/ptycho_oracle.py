@@ -40,13 +40,19 @@ def origin_and_fraction(positions: np.ndarray):
     return r0.astype(np.int64), pos - r0
 
 
-def patch_rows_cols(positions: np.ndarray, n: int, obj_shape) -> tuple:
-    """Row/col object indices of every patch pixel, periodic (dataset_models.py:514-539)."""
+def _shape2(n) -> tuple:
+    """(rows, cols) of a pattern size given as one int (square) or a pair."""
+    return (int(n), int(n)) if np.ndim(n) == 0 else (int(n[0]), int(n[1]))
+
+
+def patch_rows_cols(positions: np.ndarray, n, obj_shape) -> tuple:
+    """Row/col object indices of every patch pixel, periodic (dataset_models.py:514-539); n = N or
+    (Nr, Nc) (roi_shape)."""
+    nr, nc = _shape2(n)
     r0, _ = origin_and_fraction(positions)
-    f = fftfreq_offsets(n)
-    rows = (r0[:, 0, None] + f[None, :]) % obj_shape[-2]
-    cols = (r0[:, 1, None] + f[None, :]) % obj_shape[-1]
-    return rows, cols  # (B, n) each
+    rows = (r0[:, 0, None] + fftfreq_offsets(nr)[None, :]) % obj_shape[-2]
+    cols = (r0[:, 1, None] + fftfreq_offsets(nc)[None, :]) % obj_shape[-1]
+    return rows, cols  # (B, Nr), (B, Nc)
 
 
 def gather_patches(obj: np.ndarray, rows: np.ndarray, cols: np.ndarray) -> np.ndarray:
@@ -54,19 +60,26 @@ def gather_patches(obj: np.ndarray, rows: np.ndarray, cols: np.ndarray) -> np.nd
     return obj[:, rows[:, :, None], cols[:, None, :]]
 
 
-def translation_ramp(shifts: np.ndarray, n: int) -> np.ndarray:
-    """exp(-2 pi i kr dr) exp(-2 pi i kc dc), k = fftfreq(n, d=1) (ptycho_utils.py:156-176)."""
+def ramp_freqs(n) -> tuple:
+    """Per-axis ramp frequencies fftfreq(N, d=1) rounded to float32, as the reference builds them
+    (ptycho_utils.py:167-168); exact for powers of two, 1-ulp-of-float32 off 1/N otherwise."""
+    nr, nc = _shape2(n)
+    return (np.fft.fftfreq(nr, d=1.0).astype(np.float32).astype(np.float64),
+            np.fft.fftfreq(nc, d=1.0).astype(np.float32).astype(np.float64))
+
+
+def translation_ramp(shifts: np.ndarray, n) -> np.ndarray:
+    """exp(-2 pi i kr dr) exp(-2 pi i kc dc), k = fftfreq(N, d=1) per axis (ptycho_utils.py:156-176)."""
     s = np.asarray(shifts, dtype=np.float64)
-    k = np.fft.fftfreq(n, d=1.0)
-    rr = np.exp(-2j * np.pi * k[None, :, None] * s[:, 0, None, None])
-    rc = np.exp(-2j * np.pi * k[None, None, :] * s[:, 1, None, None])
-    return rr * rc  # (B, n, n)
+    kr, kc = ramp_freqs(n)
+    rr = np.exp(-2j * np.pi * kr[None, :, None] * s[:, 0, None, None])
+    rc = np.exp(-2j * np.pi * kc[None, None, :] * s[:, 1, None, None])
+    return rr * rc  # (B, Nr, Nc)
 
 
 def shift_probes(probe: np.ndarray, frac: np.ndarray) -> np.ndarray:
     """(M, B, n, n) = ifft2(fft2(P_m) * ramp_b) (ptycho_utils.py:128-139; probe_models.py:718-720)."""
-    n = probe.shape[-1]
-    ramp = translation_ramp(frac, n)
+    ramp = translation_ramp(frac, probe.shape[-2:])
     return np.fft.ifft2(np.fft.fft2(probe)[:, None] * ramp[None])
 
 
@@ -87,7 +100,7 @@ def multislice(patches: np.ndarray, psi0: np.ndarray, props: np.ndarray) -> tupl
     return np.stack(psis), x
 
 
-def descan_ramp(descan: np.ndarray, n: int) -> np.ndarray:
+def descan_ramp(descan: np.ndarray, n) -> np.ndarray:
     """Real-space ramp applied to the exit wave (ptychography_base.py:903-905)."""
     return translation_ramp(descan, n)
 
@@ -103,8 +116,7 @@ def batch_loss(I: np.ndarray, targets: np.ndarray, mask: Optional[np.ndarray], n
     """PtychographyBase.error_estimate (ptychography_base.py:908-932)."""
     if loss_type not in LOSS_TYPES:
         raise ValueError(f"Unknown loss type {loss_type}")
-    n = I.shape[-1]
-    m = np.ones((n, n)) if mask is None else np.asarray(mask, dtype=np.float64)
+    m = np.ones(I.shape[-2:]) if mask is None else np.asarray(mask, dtype=np.float64)
     pred = np.sqrt(I + 1e-9) if "amplitude" in loss_type else I
     B = I.shape[0]
     if loss_type == "poisson":
@@ -119,9 +131,8 @@ def batch_loss(I: np.ndarray, targets: np.ndarray, mask: Optional[np.ndarray], n
 def dloss_dI(I: np.ndarray, targets: np.ndarray, mask: Optional[np.ndarray], num_gpts: int,
              mean_intensity: float, loss_type: str = "l2_amplitude") -> np.ndarray:
     """Derivative of batch_loss with respect to the fftshifted intensities (torch's sign(0) = 0)."""
-    n = I.shape[-1]
     B = I.shape[0]
-    m = np.ones((n, n)) if mask is None else np.asarray(mask, dtype=np.float64)
+    m = np.ones(I.shape[-2:]) if mask is None else np.asarray(mask, dtype=np.float64)
     amp = "amplitude" in loss_type
     pred = np.sqrt(I + 1e-9) if amp else I
     if loss_type == "poisson":
@@ -139,7 +150,7 @@ def forward(obj, probe, positions, propagators, descan=None) -> dict:
     """Full forward: returns patches, probe shifts, waves and fftshifted intensities."""
     obj = np.asarray(obj, dtype=np.complex128)
     probe = np.asarray(probe, dtype=np.complex128)
-    n = probe.shape[-1]
+    n = probe.shape[-2:]  # (Nr, Nc)
     rows, cols = patch_rows_cols(positions, n, obj.shape)
     patches = gather_patches(obj, rows, cols)
     _, frac = origin_and_fraction(positions)
@@ -162,7 +173,8 @@ def adjoint(obj, probe, positions, propagators, targets, mask, num_gpts, mean_in
     obj = np.asarray(obj, dtype=np.complex128)
     probe = np.asarray(probe, dtype=np.complex128)
     fw = forward(obj, probe, positions, propagators, descan)
-    n = probe.shape[-1]
+    n = probe.shape[-2:]  # (Nr, Nc)
+    npix = n[0] * n[1]
     S = obj.shape[0]
     props = np.asarray(propagators, dtype=np.complex128) if S > 1 else None
     gI = dloss * dloss_dI(fw["I"], targets, mask, num_gpts, mean_intensity, loss_type)
@@ -173,13 +185,13 @@ def adjoint(obj, probe, positions, propagators, targets, mask, num_gpts, mean_in
     out = {}
     if descan is not None:
         ramp = descan_ramp(descan, n)
-        f = np.fft.fftfreq(n, d=1.0)
+        fr, fc = ramp_freqs(n)
         w = np.conj(gx) * fw["exit"]  # exit already includes the ramp
-        out["descan"] = np.stack([2 * np.pi * np.sum(f[None, None, :, None] * w.imag, axis=(0, 2, 3)),
-                                  2 * np.pi * np.sum(f[None, None, None, :] * w.imag, axis=(0, 2, 3))], axis=-1)
+        out["descan"] = np.stack([2 * np.pi * np.sum(fr[None, None, :, None] * w.imag, axis=(0, 2, 3)),
+                                  2 * np.pi * np.sum(fc[None, None, None, :] * w.imag, axis=(0, 2, 3))], axis=-1)
         gx = gx * np.conj(ramp)[None]
     g_obj = np.zeros_like(obj)
-    g_prop = np.zeros((max(S - 1, 0), n, n), dtype=np.complex128)
+    g_prop = np.zeros((max(S - 1, 0),) + tuple(n), dtype=np.complex128)
     rows, cols = fw["rows"], fw["cols"]
     for s in range(S - 1, -1, -1):
         psi = fw["psis"][s]
@@ -193,7 +205,7 @@ def adjoint(obj, probe, positions, propagators, targets, mask, num_gpts, mean_in
             # over modes and patterns)
             G = np.fft.fft2(gpsi)
             X = np.fft.fft2(fw["patches"][s - 1][None] * fw["psis"][s - 1])
-            g_prop[s - 1] = np.sum(np.conj(X) * G, axis=(0, 1)) / (n * n)
+            g_prop[s - 1] = np.sum(np.conj(X) * G, axis=(0, 1)) / npix
             gx = np.fft.ifft2(G * np.conj(props[s - 1]))
     # probe: psi0 = ifft2(fft2(P) * R) => dP = sum_b ifft2(fft2(gpsi0) * conj(R))
     ramp = translation_ramp(fw["frac"], n)
@@ -201,12 +213,12 @@ def adjoint(obj, probe, positions, propagators, targets, mask, num_gpts, mean_in
     out["probe"] = np.fft.ifft2(np.sum(G * np.conj(ramp)[None], axis=1))
     # positions: d psi0 / d frac_r = ifft2(fft2(P) R (-2 pi i kr)); <g, .> = (1/n^2) sum conj(G) Pk R (-2 pi i k)
     Pk = np.fft.fft2(probe)
-    k = np.fft.fftfreq(n, d=1.0)
+    kr, kc = ramp_freqs(n)
     Z = np.conj(G) * Pk[:, None] * ramp[None]
     out["positions"] = np.stack([
-        np.real(np.sum(Z * (-2j * np.pi * k[None, None, :, None]), axis=(0, 2, 3))),
-        np.real(np.sum(Z * (-2j * np.pi * k[None, None, None, :]), axis=(0, 2, 3))),
-    ], axis=-1) / (n * n)
+        np.real(np.sum(Z * (-2j * np.pi * kr[None, None, :, None]), axis=(0, 2, 3))),
+        np.real(np.sum(Z * (-2j * np.pi * kc[None, None, None, :]), axis=(0, 2, 3))),
+    ], axis=-1) / npix
     out["obj"] = g_obj
     out["propagators"] = g_prop
     out["loss"] = dloss * 0 + batch_loss(fw["I"], targets, mask, num_gpts, mean_intensity, loss_type)

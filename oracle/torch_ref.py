@@ -27,11 +27,12 @@ def _fftfreq(n: int, d: float, like: torch.Tensor) -> torch.Tensor:
     return torch.from_numpy(np.fft.fftfreq(n, d=d).astype(np.float32)).to(like.device)
 
 
-def patch_indices(positions: torch.Tensor, n: int, obj_shape) -> torch.Tensor:
+def patch_indices(positions: torch.Tensor, n, obj_shape) -> torch.Tensor:
+    nr, nc = (n, n) if isinstance(n, int) else n
     r0 = torch.round(positions[:, 0]).to(torch.int32)
     c0 = torch.round(positions[:, 1]).to(torch.int32)
-    x_ind = torch.fft.fftfreq(n, d=1 / n).to(positions.device)
-    y_ind = torch.fft.fftfreq(n, d=1 / n).to(positions.device)
+    x_ind = torch.fft.fftfreq(nr, d=1 / nr).to(positions.device)
+    y_ind = torch.fft.fftfreq(nc, d=1 / nc).to(positions.device)
     row = (r0[:, None, None] + x_ind[None, :, None]) % obj_shape[-2]
     col = (c0[:, None, None] + y_ind[None, None, :]) % obj_shape[-1]
     return (row * obj_shape[-1] + col).to(torch.int32)
@@ -57,7 +58,7 @@ def fourier_shift_expand(array: torch.Tensor, positions: torch.Tensor) -> torch.
 def batch_loss(obj, probe, positions, propagators, targets, mask, num_gpts, mean_intensity,
                loss_type="l2_amplitude", descan: Optional[torch.Tensor] = None):
     """Returns (loss, pred_intensities) following the reference op order."""
-    n = probe.shape[-1]
+    n = tuple(probe.shape[-2:])  # roi_shape (Nr, Nc)
     frac = positions - torch.round(positions)
     with torch.no_grad():
         idx = patch_indices(positions.detach(), n, obj.shape)
@@ -72,14 +73,14 @@ def batch_loss(obj, probe, positions, propagators, targets, mask, num_gpts, mean
         propagated.append(p)
     torch.stack(propagated, dim=0)  # the reference materialises this stack every batch
     if descan is not None:
-        overlap = overlap * fourier_translation_operator(descan, (n, n), overlap.dtype)[None]
+        overlap = overlap * fourier_translation_operator(descan, n, overlap.dtype)[None]
     exit_fft = torch.fft.fft2(overlap, norm="ortho")
     intens = torch.fft.fftshift(torch.sum(torch.abs(exit_fft) ** 2, dim=0), dim=(-2, -1))
     if "amplitude" in loss_type:
         preds = torch.sqrt(intens + 1e-9)
     else:
         preds = intens
-    m = torch.ones((n, n), dtype=preds.dtype) if mask is None else mask
+    m = torch.ones(n, dtype=preds.dtype) if mask is None else mask
     diff = preds * m - targets * m
     B = diff.shape[0]
     if "l1" in loss_type:

@@ -272,9 +272,10 @@ def forward_intensities(
     return intens
 
 
-def fresnel_propagators(roi: int, sampling: float, energy: float, thicknesses, tilt_mrad: torch.Tensor,
+def fresnel_propagators(roi, sampling: float, energy: float, thicknesses, tilt_mrad: torch.Tensor,
                         device=None) -> torch.Tensor:
-    """(S-1, N, N) complex64 Fresnel propagators with a probe tilt, differentiable in ``tilt_mrad``.
+    """(S-1, Nr, Nc) complex64 Fresnel propagators with a probe tilt, differentiable in ``tilt_mrad``
+    (``roi`` = N or (Nr, Nc)).
 
     The reference's ``ProbeBase._compute_propagator_arrays`` (``probe_models.py:376-427``), op for op:
     exp(-i pi lambda dz k^2) times, for a tilt component that is not exactly zero,
@@ -284,8 +285,9 @@ def fresnel_propagators(roi: int, sampling: float, energy: float, thicknesses, t
     from .synth import electron_wavelength_angstrom
 
     dev = tilt_mrad.device if device is None else torch.device(device)
-    kr = torch.fft.fftfreq(roi, sampling, device=dev)
-    kc = torch.fft.fftfreq(roi, sampling, device=dev)
+    nr, nc = (roi, roi) if isinstance(roi, int) else (int(roi[0]), int(roi[1]))
+    kr = torch.fft.fftfreq(nr, sampling, device=dev)
+    kc = torch.fft.fftfreq(nc, sampling, device=dev)
     k2 = (kr[:, None] ** 2 + kc[None] ** 2).to(torch.complex64)
     wavelength = electron_wavelength_angstrom(energy)
     dz = torch.tensor(list(thicknesses), device=dev, dtype=k2.dtype)

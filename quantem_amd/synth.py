@@ -49,19 +49,26 @@ def gaussian_atom_phase(shape, n_atoms: int, sigma: float, amp_range, rng: np.ra
     return phi
 
 
-def fourier_probe(n: int, sampling: float, energy: float, semiangle_mrad: float, defocus: float,
+def _shape2(n) -> tuple:
+    """(rows, cols) of a pattern size given as one int (square) or a pair (roi_shape)."""
+    return (int(n), int(n)) if np.ndim(n) == 0 else (int(n[0]), int(n[1]))
+
+
+def fourier_probe(n, sampling: float, energy: float, semiangle_mrad: float, defocus: float,
                   soft_edges: bool = True) -> np.ndarray:
-    """Aperture * exp(-i chi) in corner-centred Fourier space, unit L2 norm."""
+    """Aperture * exp(-i chi) in corner-centred Fourier space, unit L2 norm; n = N or (Nr, Nc)."""
+    nr, nc = _shape2(n)
     lam = electron_wavelength_angstrom(energy)
-    k = np.fft.fftfreq(n, sampling)
-    kx, ky = np.meshgrid(k, k, indexing="ij")
+    k_r, k_c = np.fft.fftfreq(nr, sampling), np.fft.fftfreq(nc, sampling)
+    kx, ky = np.meshgrid(k_r, k_c, indexing="ij")
     kr = np.sqrt(kx**2 + ky**2)
     alpha = kr * lam
     phi = np.arctan2(ky, kx)
     cut = semiangle_mrad * 1e-3
     if soft_edges:
-        da_r = (k[1] - k[0]) * lam  # angular sampling (rad) along each axis
-        denom = np.sqrt((np.cos(phi) * da_r) ** 2 + (np.sin(phi) * da_r) ** 2)
+        # angular sampling (rad) along each axis (complex_probe.py:449, 51-86)
+        da_r, da_c = (k_r[1] - k_r[0]) * lam, (k_c[1] - k_c[0]) * lam
+        denom = np.sqrt((np.cos(phi) * da_r) ** 2 + (np.sin(phi) * da_c) ** 2)
         ap = np.clip((cut - alpha) / denom + 0.5, 0.0, 1.0)
     else:
         ap = (alpha <= cut).astype(np.float64)
@@ -71,7 +78,7 @@ def fourier_probe(n: int, sampling: float, energy: float, semiangle_mrad: float,
     return f / np.sqrt(np.sum(np.abs(f) ** 2))
 
 
-def real_space_probe(n: int, sampling: float, energy: float, semiangle_mrad: float, defocus: float) -> np.ndarray:
+def real_space_probe(n, sampling: float, energy: float, semiangle_mrad: float, defocus: float) -> np.ndarray:
     p = np.fft.ifft2(fourier_probe(n, sampling, energy, semiangle_mrad, defocus))
     return p / np.sqrt(np.sum(np.abs(p) ** 2))
 
@@ -96,10 +103,10 @@ def mixed_probe(base: np.ndarray, weights: Sequence[float], total_intensity: flo
     sum_m sum |fft2_ortho(P_m)|^2 = total_intensity and the per-mode power follows `weights`
     (the scaling of probe_models.py:860-873).
     """
-    n = base.shape[-1]
+    nr, nc = base.shape[-2:]
     M = len(weights)
-    f = np.fft.fftfreq(n, 1.0 / n)  # corner-centred pixel offsets
-    y, x = np.meshgrid(f, f, indexing="ij")
+    # corner-centred pixel offsets along rows (y) and columns (x)
+    y, x = np.meshgrid(np.fft.fftfreq(nr, 1.0 / nr), np.fft.fftfreq(nc, 1.0 / nc), indexing="ij")
     w0 = max(np.sqrt(np.sum(np.abs(base) ** 2 * (x**2 + y**2)) / np.sum(np.abs(base) ** 2)), 1.0)
     u, v = x / w0, y / w0
     factors = [np.ones_like(u), u, v, u * v, u**2 - 0.5, v**2 - 0.5, u**3, v**3]
@@ -120,25 +127,26 @@ def raster_positions(ny: int, nx: int, step: float, jitter: float, rng: Optional
     return pos
 
 
-def fresnel_propagators(n: int, sampling: float, energy: float, thicknesses: Sequence[float],
+def fresnel_propagators(n, sampling: float, energy: float, thicknesses: Sequence[float],
                         tilt_mrad=(0.0, 0.0)) -> np.ndarray:
-    """(S-1, n, n) propagators exp(-i pi lambda dz k^2) [* tilt ramps], fftfreq order (complex64).
-
-    Evaluated the way probe_models.py:376-427 does: k^2 and the phase in complex64.
+    """(S-1, Nr, Nc) propagators exp(-i pi lambda dz k^2) [* tilt ramps], fftfreq order (complex64);
+    n = N or (Nr, Nc).  Evaluated the way probe_models.py:376-427 does: k^2 and the phase in complex64.
     """
+    nr, nc = _shape2(n)
     if len(thicknesses) == 0:
-        return np.zeros((0, n, n), dtype=np.complex64)
+        return np.zeros((0, nr, nc), dtype=np.complex64)
     lam = electron_wavelength_angstrom(energy)
-    k = np.fft.fftfreq(n, sampling).astype(np.float32)
-    k2 = (k[:, None] ** 2 + k[None, :] ** 2).astype(np.complex64)
+    kr = np.fft.fftfreq(nr, sampling).astype(np.float32)
+    kc = np.fft.fftfreq(nc, sampling).astype(np.float32)
+    k2 = (kr[:, None] ** 2 + kc[None, :] ** 2).astype(np.complex64)
     dz = np.asarray(thicknesses, dtype=np.complex64)
     phase = (np.complex64(-1j * np.pi * lam) * dz)[:, None, None]
     prop = np.exp((phase * k2[None]).astype(np.complex64)).astype(np.complex64)
     tr, tc = tilt_mrad
     if tr != 0:
-        prop = prop * np.exp(1j * (-2 * np.pi * dz.real[:, None, None] * np.tan(tr / 1e3)) * k[None, :, None])
+        prop = prop * np.exp(1j * (-2 * np.pi * dz.real[:, None, None] * np.tan(tr / 1e3)) * kr[None, :, None])
     if tc != 0:
-        prop = prop * np.exp(1j * (-2 * np.pi * dz.real[:, None, None] * np.tan(tc / 1e3)) * k[None, None, :])
+        prop = prop * np.exp(1j * (-2 * np.pi * dz.real[:, None, None] * np.tan(tc / 1e3)) * kc[None, None, :])
     return prop.astype(np.complex64)
 
 

@@ -11,12 +11,13 @@ def make_case(N=32, S=3, M=2, B=6, H=None, W=None, seed=0, descan=False, half_in
               dz=2.0, loss_type="l2_amplitude", weak=False):
     """Random multislice problem with realistic probe modes and a non-trivial loss."""
     rng = np.random.default_rng(seed)
-    H = H or N + 24
-    W = W or N + 16
+    nr, nc = (N, N) if np.ndim(N) == 0 else (int(N[0]), int(N[1]))  # roi_shape
+    H = H or nr + 24
+    W = W or nc + 16
     phase = rng.uniform(-0.6, 0.6, (S, H, W))
     amp = rng.uniform(0.6, 1.0, (S, H, W)) if not weak else np.ones((S, H, W))
     obj = (amp * np.exp(1j * phase)).astype(np.complex64)
-    samp = sampling or (2.0 / N)  # keeps the 20 mrad aperture inside the band for any N
+    samp = sampling or (2.0 / min(nr, nc))  # keeps the 20 mrad aperture inside the band for any N
     base = synth.real_space_probe(N, samp, 300e3, 20.0, 40.0)
     weights = [0.7, 0.15, 0.1, 0.05, 0.03, 0.02][:M]
     probe = synth.mixed_probe(base, weights, 1e3).astype(np.complex64)
@@ -28,7 +29,7 @@ def make_case(N=32, S=3, M=2, B=6, H=None, W=None, seed=0, descan=False, half_in
     pos = pos.astype(np.float32)
     props = synth.fresnel_propagators(N, samp, 300e3, [dz] * (S - 1))
     d = rng.uniform(-1.5, 1.5, (B, 2)).astype(np.float32) if descan else None
-    mask = rng.uniform(0.5, 1.0, (N, N)).astype(np.float32)
+    mask = rng.uniform(0.5, 1.0, (nr, nc)).astype(np.float32)
     return {"obj": obj, "probe": probe, "positions": pos, "propagators": props, "descan": d, "mask": mask,
             "num_gpts": 4 * B, "rng": rng, "loss_type": loss_type}
 

@@ -278,3 +278,41 @@ def test_analytical_potential_branch_is_the_phase_chain_rule():
     assert np.abs(og_v.imag).max() == 0.0
     assert rel(og_v.real, np.real(-1j * np.conj(o128) * og_c)) < 1e-12
     assert rel(pg_v, pg_c) < 1e-14
+
+
+def test_rectangular_patterns_adjoint_matches_autograd_and_finite_differences():
+    """Non-square, non-power-of-two patterns (roi_shape (12, 20)): the oracle's gradients equal torch
+    autograd of the reference-order restatement (complex128) and complex128 finite differences, and its
+    patch indices follow fftfreq offsets per axis."""
+    case = make_case(N=(12, 20), S=3, M=2, B=3, seed=12, descan=True, half_integer=False)
+    I = po.forward(case["obj"], case["probe"], case["positions"], case["propagators"], case["descan"])["I"]
+    assert I.shape == (3, 12, 20)
+    case = attach_targets(case, I)
+    a = po.adjoint(case["obj"], case["probe"], case["positions"], case["propagators"], case["targets"],
+                   case["mask"], case["num_gpts"], case["mean_intensity"], descan=case["descan"])
+    b = torch_ref.loss_and_grads(case["obj"], case["probe"], case["positions"], case["propagators"],
+                                 case["targets"], case["mask"], case["num_gpts"], case["mean_intensity"],
+                                 descan=case["descan"])
+    assert abs(a["loss"] - b["loss"]) <= 1e-12 * abs(b["loss"])
+    for k in ("obj", "probe", "positions", "descan", "propagators"):
+        assert rel(a[k], b[k]) < 1e-10, k
+    rows, cols = po.patch_rows_cols(case["positions"], (12, 20), case["obj"].shape)
+    r0 = np.round(case["positions"]).astype(int)
+    assert np.array_equal(rows[0], (r0[0, 0] + np.r_[0:6, -6:0]) % case["obj"].shape[1])
+    assert np.array_equal(cols[0], (r0[0, 1] + np.r_[0:10, -10:0]) % case["obj"].shape[2])
+    obj = case["obj"].astype(np.complex128)
+    props = case["propagators"].astype(np.complex128)
+
+    def L(o):
+        I = po.forward(o, case["probe"], case["positions"], props, case["descan"])["I"]
+        return po.batch_loss(I, case["targets"], case["mask"], case["num_gpts"], case["mean_intensity"])
+
+    rng = np.random.default_rng(1)
+    h = 1e-4
+    for _ in range(6):
+        s, bb = rng.integers(0, 3), rng.integers(0, 3)
+        i, j = rows[bb, rng.integers(0, 12)], cols[bb, rng.integers(0, 20)]
+        e = np.zeros_like(obj)
+        e[s, i, j] = h
+        fd = (L(obj + e) - L(obj - e)) / (2 * h)
+        assert fd == pytest.approx(a["obj"][s, i, j].real, rel=1e-5, abs=1e-9)
