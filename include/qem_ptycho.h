@@ -18,6 +18,9 @@
  *   targets      (B, N, N)    float32    measured amplitudes (or intensities), fftshifted (DC at N/2, N/2)
  *   mask         (N, N)       float32    detector mask, fftshifted
  *   descan       (B, 2)       float32    optional descan shifts in pixels (NULL = none)
+ * (N, N) stands for the pattern shape (Nr, Nc) = (roi, roi_cols) of the descriptor; fftshift moves index k
+ * of an axis of length n to (k + n / 2) mod n, and patch pixel k lies at offset fftfreq(n, 1/n)[k] from the
+ * origin (dataset_models.py:514-539, per axis).
  *
  * Every call is asynchronous on `stream` (a hipStream_t passed as void*), allocates nothing,
  * retains no pointer past its return, and gives results that do not depend on launch order
@@ -67,10 +70,10 @@ enum {
 typedef struct qem_ptycho_desc {
   int32_t num_slices;      /* S  >= 1 */
   int32_t num_modes;       /* M  >= 1 */
-  int32_t batch;           /* B  >= 1, patterns in this call; B * M * roi^2 < 2^31 */
-  int32_t roi;             /* N  in {32, 64, 128, 256} */
-  int32_t obj_rows;        /* Ho >= N */
-  int32_t obj_cols;        /* Wo >= N */
+  int32_t batch;           /* B  >= 1, patterns in this call; B * M * Nr * Nc < 2^31 */
+  int32_t roi;             /* Nr, pattern rows (roi_shape[0], dataset_models.py:519) */
+  int32_t obj_rows;        /* Ho >= Nr */
+  int32_t obj_cols;        /* Wo >= Nc */
   int32_t num_gpts;        /* P  total scan positions of the dataset (loss normalisation) */
   int32_t loss_type;       /* QEM_LOSS_* */
   float mean_intensity;    /* mean diffraction intensity of the dataset (> 0) */
@@ -81,6 +84,11 @@ typedef struct qem_ptycho_desc {
                               time (workspace B*M*(ceil(S/k) + k)*N*N*8 bytes instead of B*M*S*N*N*8,
                               one extra forward propagation per slice).  No reference counterpart:
                               torch autograd keeps every intermediate (ptychography_base.py:934-947). */
+  int32_t roi_cols;        /* Nc, pattern columns (roi_shape[1]); 0 = Nc equal to Nr.  Square 32, 64, 128
+                              and 256 run the fixed-size kernels; any other Nr x Nc with 2 <= Nr, Nc <= 512
+                              and prime factors 2, 3, 5, 7 only runs the generic-size kernels (mixed-radix
+                              LDS FFTs; no qem_ptycho_projection_update there); other shapes return
+                              QEM_ERR_UNSUPPORTED. */
 } qem_ptycho_desc;
 
 /* Bytes of device workspace a forward_loss/adjoint pair needs for this descriptor.
@@ -163,9 +171,9 @@ int qem_ptycho_projection_update(const qem_ptycho_desc* desc, const void* obj, c
                                  void* obj_update, void* probe_update, float* probe_overlap, float* patch_energy,
                                  void* workspace, size_t workspace_bytes, void* stream);
 
-/* Batched 2-D complex64 FFT of `count` N x N arrays with torch.fft.fft2 / ifft2 default
- * normalisation (forward unscaled, inverse 1/N^2).  Replaces af.fft2 / af.ifft2
- * (core/utils/array_funcs.py:263-285) for the square power-of-two sizes above. */
+/* Batched 2-D complex64 FFT of `count` N x N arrays (N in {32, 64, 128}) with torch.fft.fft2 / ifft2
+ * default normalisation (forward unscaled, inverse 1/N^2).  Replaces af.fft2 / af.ifft2
+ * (core/utils/array_funcs.py:263-285) for those sizes. */
 int qem_fft2_c64(int32_t n, int32_t count, int32_t inverse, const void* in, void* out, void* stream);
 
 /* Measurement hook (no reference counterpart; the reference has no timing code): while active,
@@ -179,7 +187,7 @@ int qem_profile_end(float* fwd_ms, int32_t* n_fwd, float* adj_ms, int32_t* n_adj
 /* Thread-local description of the last error (empty string if none). */
 const char* qem_last_error(void);
 
-/* ABI version (major * 100 + minor): 102. */
+/* ABI version (major * 100 + minor): 103. */
 int32_t qem_abi_version(void);
 
 #ifdef __cplusplus

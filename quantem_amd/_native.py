@@ -36,7 +36,24 @@ LOSS_TYPES = {
     "poisson": 4,
 }
 
-SUPPORTED_ROI = (32, 64, 128, 256)  # multislice pattern sizes
+SUPPORTED_ROI = (32, 64, 128, 256)  # square pattern sizes of the fixed-size multislice kernels
+GENERIC_ROI_MAX = 512  # generic-size kernels: Nr x Nc, 2..512 per axis, prime factors 2, 3, 5, 7
+
+
+def roi_supported(nr: int, nc: int) -> bool:
+    """Whether the library runs Nr x Nc patterns (the fixed-size square kernels or the generic ones)."""
+    if nr == nc and nr in SUPPORTED_ROI:
+        return True
+
+    def smooth(n: int) -> bool:
+        if n < 2 or n > GENERIC_ROI_MAX:
+            return False
+        for r in (2, 3, 5, 7):
+            while n % r == 0:
+                n //= r
+        return n == 1
+
+    return smooth(nr) and smooth(nc)
 FFT_ROI = (32, 64, 128)  # sizes of the standalone fft2 entry point
 
 EXPORTED_SYMBOLS = (
@@ -83,6 +100,7 @@ class QemDesc(ctypes.Structure):
         ("mean_intensity", ctypes.c_float),
         ("flags", ctypes.c_uint32),
         ("segment_slices", ctypes.c_int32),
+        ("roi_cols", ctypes.c_int32),
     ]
 
 

@@ -19,7 +19,8 @@ LIB_DIR = os.path.join(PKG_DIR, "_lib")
 LIB_PATH = os.path.join(LIB_DIR, "libqem_ptycho.so")
 SOURCES = [os.path.join(CSRC, "ptycho.hip"), os.path.join(CSRC, "ptycho256.hip"), os.path.join(CSRC, "nn.hip"),
            os.path.join(CSRC, "optim.hip"), os.path.join(CSRC, "constraints.hip"),
-           os.path.join(CSRC, "preprocess.hip"), os.path.join(CSRC, "probe.hip")]
+           os.path.join(CSRC, "preprocess.hip"), os.path.join(CSRC, "probe.hip"),
+           os.path.join(CSRC, "ptycho_gen.hip")]
 HEADERS = [
     os.path.join(CSRC, "fft2.h"),
     os.path.join(CSRC, "twiddles.h"),

@@ -1002,9 +1002,16 @@ struct Layout {
   int ck, nck; // checkpoint interval (0 = every incident wave is stored) and checkpoints per item
 };
 
+// Pattern columns (roi_cols, 0 = square) and whether the generic-size kernels run the descriptor.
+int cols_of(const qem_ptycho_desc& d) { return d.roi_cols != 0 ? d.roi_cols : d.roi; }
+bool fixed_size(const qem_ptycho_desc& d) {
+  return cols_of(d) == d.roi && (d.roi == 32 || d.roi == 64 || d.roi == 128 || d.roi == 256);
+}
+
 Layout layout_of(const qem_ptycho_desc& d) {
   Layout L;
-  const size_t N2 = (size_t)d.roi * d.roi;
+  const size_t N2 = (size_t)d.roi * cols_of(d);
+  const bool two_pass = d.roi == 256 || !fixed_size(d);  // chains with a per-item row-transformed wave
   const size_t BM = (size_t)d.batch * d.num_modes;
   const int S = d.num_slices;
   const bool store = (d.flags & QEM_FLAG_STORE_WAVES) != 0;
@@ -1018,12 +1025,12 @@ Layout layout_of(const qem_ptycho_desc& d) {
   L.ckpt = o;
   o += align_up(BM * L.nck * N2 * 8);
   L.scratch = o;
-  if (d.roi == 256) o += align_up(BM * N2 * 8);  // the row-transformed wave of each item
+  if (two_pass) o += align_up(BM * N2 * 8);  // the row-transformed wave of each item
   L.X = o;
   o += align_up(BM * N2 * 8);
   L.gI = o;
   o += align_up((size_t)d.batch * N2 * 4);
-  L.nloss = (int)((size_t)d.batch * N2 / 256);
+  L.nloss = (int)(((size_t)d.batch * N2 + 255) / 256);
   L.loss_part = o;
   o += align_up((size_t)L.nloss * 8);
   L.pk = o;
@@ -1035,7 +1042,7 @@ Layout layout_of(const qem_ptycho_desc& d) {
   L.gdesc = o;
   o += align_up(BM * 2 * 4);
   L.ramps = o;
-  o += align_up((size_t)d.batch * 2 * d.roi * 8);
+  o += align_up((size_t)d.batch * (d.roi + cols_of(d)) * 8);
   L.pgrad = store && (d.flags & QEM_FLAG_GRAD_PROPAGATORS) && S > 1;
   L.pspec = o;
   if (L.pgrad) o += align_up(BM * (size_t)(S - 1) * N2 * 8);
@@ -1045,31 +1052,37 @@ Layout layout_of(const qem_ptycho_desc& d) {
 
 // The translation units that share qem::Params agree on its layout (see params_signature_local).
 int check_params_layout() {
-  static const bool same = qem::params_signature_local() == qem::params_signature_256();
+  static const bool same = qem::params_signature_local() == qem::params_signature_256() &&
+                           qem::params_signature_local() == qem::params_signature_gen();
   if (!same)
     return fail(QEM_ERR_LAUNCH,
-                "library build error: qem::Params has a different layout in ptycho.hip and ptycho256.hip "
-                "(translation units compiled from different headers); rebuild the library");
+                "library build error: qem::Params has a different layout in ptycho.hip, ptycho256.hip and "
+                "ptycho_gen.hip (translation units compiled from different headers); rebuild the library");
   return QEM_OK;
 }
 
 int check_desc(const qem_ptycho_desc* d) {
   if (int rc = check_params_layout()) return rc;
   if (d == nullptr) return fail(QEM_ERR_NULL, "desc is NULL");
-  if (d->roi != 32 && d->roi != 64 && d->roi != 128 && d->roi != 256)
-    return fail(QEM_ERR_UNSUPPORTED, "roi %d not supported (32, 64, 128 or 256)", d->roi);
+  if (d->roi_cols < 0) return fail(QEM_ERR_ARG, "roi_cols must be >= 0 (got %d)", d->roi_cols);
+  const int nc = cols_of(*d);
+  if (!fixed_size(*d) && !qem::gen_supported(d->roi, nc))
+    return fail(QEM_ERR_UNSUPPORTED,
+                "pattern shape (%d, %d) not supported: square 32, 64, 128 or 256, or 2..512 per axis with "
+                "prime factors 2, 3, 5 and 7 only",
+                d->roi, nc);
   if (d->num_slices < 1) return fail(QEM_ERR_ARG, "num_slices must be >= 1 (got %d)", d->num_slices);
   if (d->num_modes < 1) return fail(QEM_ERR_ARG, "num_modes must be >= 1 (got %d)", d->num_modes);
   if (d->batch < 1) return fail(QEM_ERR_ARG, "batch must be >= 1 (got %d)", d->batch);
-  if (d->obj_rows < d->roi || d->obj_cols < d->roi)
-    return fail(QEM_ERR_ARG, "object (%d, %d) must be at least roi %d in each dimension", d->obj_rows, d->obj_cols,
-                d->roi);
+  if (d->obj_rows < d->roi || d->obj_cols < nc)
+    return fail(QEM_ERR_ARG, "object (%d, %d) must be at least the pattern (%d, %d) in each dimension", d->obj_rows,
+                d->obj_cols, d->roi, nc);
   if ((size_t)d->num_slices * d->obj_rows * d->obj_cols >= ((size_t)1 << 31) ||
       (size_t)d->obj_rows * d->obj_cols > ((size_t)1 << 28))
     return fail(QEM_ERR_ARG, "object has too many elements (at most 2^28 per slice, 2^31 in total)");
-  if ((size_t)d->batch * d->num_modes * d->roi * d->roi >= ((size_t)1 << 31))
-    return fail(QEM_ERR_ARG, "batch * num_modes * roi^2 must stay below 2^31 (batch %d, %d modes, roi %d)", d->batch,
-                d->num_modes, d->roi);
+  if ((size_t)d->batch * d->num_modes * d->roi * nc >= ((size_t)1 << 31))
+    return fail(QEM_ERR_ARG, "batch * num_modes * Nr * Nc must stay below 2^31 (batch %d, %d modes, pattern %d x %d)",
+                d->batch, d->num_modes, d->roi, nc);
   if (d->num_gpts < 1) return fail(QEM_ERR_ARG, "num_gpts must be >= 1 (got %d)", d->num_gpts);
   if (d->segment_slices < 0) return fail(QEM_ERR_ARG, "segment_slices must be >= 0 (got %d)", d->segment_slices);
   if ((d->flags & QEM_FLAG_GRAD_PROPAGATORS) && d->segment_slices >= 2 && d->segment_slices < d->num_slices)
@@ -1097,6 +1110,8 @@ qem::Params make_params(const qem_ptycho_desc& d, const Layout& L, void* ws) {
   p.B = d.batch;
   p.H = d.obj_rows;
   p.W = d.obj_cols;
+  p.NR = d.roi;
+  p.NC = cols_of(d);
   p.P = d.num_gpts;
   p.loss_type = d.loss_type;
   p.mean_I = d.mean_intensity;
@@ -1111,7 +1126,7 @@ qem::Params make_params(const qem_ptycho_desc& d, const Layout& L, void* ws) {
   p.s1 = d.num_slices;
   p.fwd_mode = !(d.flags & QEM_FLAG_STORE_WAVES) ? qem::FWD_NO_STORE
                                                  : (L.ck ? qem::FWD_CHECKPOINT : qem::FWD_STORE_ALL);
-  p.scratch = (d.roi == 256) ? (qem::cf*)(w + L.scratch) : nullptr;
+  p.scratch = (d.roi == 256 || !fixed_size(d)) ? (qem::cf*)(w + L.scratch) : nullptr;
   p.X = (qem::cf*)(w + L.X);
   p.gI = (float*)(w + L.gI);
   p.loss_part = (double*)(w + L.loss_part);
@@ -1176,6 +1191,15 @@ int launch_forward256(const qem::Params& p, const Layout& L, hipStream_t st) {
   return check_launch("loss_finalize");
 }
 
+int launch_forward_gen(const qem::Params& p, const Layout& L, hipStream_t st) {
+  prof_mark(false, 0, st);
+  if (int rc = qem::launch_gen_forward(p, st)) return rc;
+  prof_mark(false, 1, st);
+  if (int rc = qem::launch_gen_loss(p, L.nloss, st)) return rc;
+  hipLaunchKernelGGL(qem::k_loss_finalize, dim3(1), dim3(256), 0, st, p, qem::gen_loss_parts(L.nloss));
+  return check_launch("loss_finalize");
+}
+
 int launch_reductions(const qem::Params& p, hipStream_t st);
 
 // Slice segments of the adjoint, last first: [j ck, min(S, (j + 1) ck)) for j = nck-1 .. 0, or one
@@ -1227,6 +1251,22 @@ int launch_adjoint256(const qem::Params& p, hipStream_t st) {
   hipLaunchKernelGGL(qem::k_probe_reduce<256>, dim3(p.M * 256 * 256 / 64), dim3(256), 0, st, p);
   if (int rc = check_launch("probe_reduce")) return rc;
   if (int rc = qem::launch_probe_grad256(p, st)) return rc;
+  return launch_reductions(p, st);
+}
+
+int launch_adjoint_gen(const qem::Params& p, hipStream_t st) {
+  prof_mark(true, 0, st);
+  int rc = for_each_segment(p, [&](const qem::Params& q) {
+    if (q.fwd_mode == qem::FWD_RECOMPUTE)
+      if (int r = qem::launch_gen_forward_kernel(q, st)) return r;
+    if (int r = qem::launch_gen_adjoint(q, st)) return r;
+    return q.ck ? qem::launch_gen_obj_reduce(q, st) : QEM_OK;
+  });
+  prof_mark(true, 1, st);
+  if (rc) return rc;
+  if (p.ck == 0)
+    if (int r = qem::launch_gen_obj_reduce(p, st)) return r;
+  if (int r = qem::launch_gen_probe_grad(p, st)) return r;
   return launch_reductions(p, st);
 }
 
@@ -1374,7 +1414,7 @@ int forward_loss_checked(const qem_ptycho_desc& d, const void* obj, const void* 
 
 extern "C" {
 
-int32_t qem_abi_version(void) { return 102; }
+int32_t qem_abi_version(void) { return 103; }
 
 int qem_profile_begin(int32_t max_launches) {
   qem::clear_error();
@@ -1509,6 +1549,7 @@ int forward_loss_checked(const qem_ptycho_desc& d, const void* obj, const void* 
   p.loss_out = loss;
   p.intens_out = intensities;
   hipStream_t st = (hipStream_t)stream;
+  if (!fixed_size(d)) return launch_forward_gen(p, L, st);
   switch (d.roi) {
     case 32: return launch_forward<32>(p, L, st);
     case 64: return launch_forward<64>(p, L, st);
@@ -1566,6 +1607,7 @@ int qem_ptycho_adjoint(const qem_ptycho_desc* desc, const void* obj, const void*
   p.grad_pos = grad_positions;
   p.grad_descan = grad_descan;
   hipStream_t st = (hipStream_t)stream;
+  if (!fixed_size(d)) return launch_adjoint_gen(p, st);
   switch (d.roi) {
     case 32: return launch_adjoint<32>(p, st);
     case 64: return launch_adjoint<64>(p, st);
@@ -1588,10 +1630,11 @@ int qem_ptycho_propagator_gradient(const qem_ptycho_desc* desc, void* grad_propa
   if (workspace_bytes < L.total)
     return fail(QEM_ERR_WORKSPACE, "workspace has %zu bytes, needs %zu", workspace_bytes, L.total);
   const qem::Params p = make_params(d, L, workspace);
-  const int N = d.roi;
-  const int blocks = (int)((size_t)(d.num_slices - 1) * N * N / 64);
   hipStream_t st = (hipStream_t)stream;
   qem::cf* g = (qem::cf*)grad_propagators;
+  if (!fixed_size(d)) return qem::launch_gen_prop_reduce(p, g, st);
+  const int N = d.roi;
+  const int blocks = (int)((size_t)(d.num_slices - 1) * N * N / 64);
   switch (N) {
     case 32: hipLaunchKernelGGL(qem::k_prop_reduce<32>, dim3(blocks), dim3(256), 0, st, p, g); break;
     case 64: hipLaunchKernelGGL(qem::k_prop_reduce<64>, dim3(blocks), dim3(256), 0, st, p, g); break;
@@ -1612,6 +1655,9 @@ int qem_ptycho_projection_update(const qem_ptycho_desc* desc, const void* obj, c
     return fail(QEM_ERR_ARG, "projection update needs a forward pass run with QEM_FLAG_STORE_WAVES");
   if (layout_of(d).ck != 0)
     return fail(QEM_ERR_UNSUPPORTED, "projection update needs every incident wave stored (segment_slices 0)");
+  if (!fixed_size(d))
+    return fail(QEM_ERR_UNSUPPORTED, "projection update runs for square 32, 64, 128 and 256 patterns only (got %d x %d)",
+                d.roi, cols_of(d));
   if (int rc = check_ptr(obj, "obj", true)) return rc;
   if (int rc = check_ptr(probe, "probe", true)) return rc;
   if (int rc = check_ptr(positions, "positions", false)) return rc;

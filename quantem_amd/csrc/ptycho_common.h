@@ -1,5 +1,6 @@
-// ptycho_common.h — parameters and device helpers shared by the N <= 128 (register/LDS) and the
-// N = 256 (two-pass) multislice kernels.
+// ptycho_common.h — parameters and device helpers shared by the N <= 128 (register/LDS), the
+// N = 256 (two-pass) and the generic-size (Nr x Nc, ptycho_gen.hip) multislice kernels.  Buffer shapes
+// below write (N, N) for one pattern; the generic chain stores (Nr, Nc) in natural order there.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -13,6 +14,7 @@ struct Params {
   int layout_test_extra;  // only in the CPU test build that checks the layout comparison below
 #endif
   int S, M, B, H, W, P, loss_type;
+  int NR, NC;  // pattern rows and columns (NR == NC == N for the power-of-two kernels)
   float mean_I;
   unsigned flags;
   const cf* obj;
@@ -37,7 +39,8 @@ struct Params {
   double* loss_part;  // per loss block
   cf* pk;           // (M, N, N)      FFT2 of the probe modes
   cf* gsum;         // (M, N, N)      sum over the batch of probe-gradient spectra
-  cf* ramps;        // (B, 2, N)      sub-pixel shift ramps exp(-2 pi i f dr_b), exp(-2 pi i f dc_b), f = fftfreq(N)
+  cf* ramps;        // (B, Nr + Nc)   sub-pixel shift ramps exp(-2 pi i f dr_b) (f = fftfreq(Nr)), then
+                    //                exp(-2 pi i f dc_b) (f = fftfreq(Nc)); (B, 2, N) when square
   float* gpos_part;   // (B*M, 2)
   float* gdesc_part;  // (B*M, 2)
   // slice range and wave storage of one launch.  psi holds `wslots` slots per item; slice s of the
@@ -56,13 +59,13 @@ struct Params {
 };
 
 // Layout signature of Params as compiled in the current translation unit: its size and every field's
-// offset folded into one value.  Params crosses from ptycho.hip (the C ABI) into ptycho256.hip (the
-// N = 256 launchers) by reference, so both must agree on it; a build that compiled the two with
-// different versions of this header would hand the 256 kernels a struct whose pointers sit at other
-// offsets.  The entry points compare params_signature_local() with params_signature_256() and refuse
-// to run on a mismatch.
+// offset folded into one value.  Params crosses from ptycho.hip (the C ABI) into ptycho256.hip and
+// ptycho_gen.hip (the N = 256 and generic-size launchers) by reference, so all three must agree on it;
+// a build that compiled them with different versions of this header would hand the kernels a struct
+// whose pointers sit at other offsets.  The entry points compare params_signature_local() with
+// params_signature_256() and params_signature_gen() and refuse to run on a mismatch.
 #define QEM_PARAMS_FIELDS(QF)                                                                          \
-  QF(S) QF(M) QF(B) QF(H) QF(W) QF(P) QF(loss_type) QF(mean_I) QF(flags) QF(obj) QF(probe) QF(pos) QF(prop)        \
+  QF(S) QF(M) QF(B) QF(H) QF(W) QF(P) QF(loss_type) QF(NR) QF(NC) QF(mean_I) QF(flags) QF(obj) QF(probe) QF(pos) QF(prop)        \
   QF(targets) QF(mask) QF(descan) QF(loss_out) QF(intens_out) QF(dloss) QF(grad_obj) QF(grad_probe)           \
   QF(grad_pos) QF(grad_descan) QF(psi) QF(scratch) QF(X) QF(gI) QF(loss_part) QF(pk) QF(gsum) QF(ramps)          \
   QF(gpos_part) QF(gdesc_part) QF(s0) QF(s1) QF(wslots) QF(ck) QF(nck) QF(fwd_mode) QF(ckpt) QF(pspec)
@@ -74,8 +77,9 @@ static constexpr unsigned long long params_signature_local() {
 #undef QEM_PARAMS_FOLD
   return h;
 }
-// params_signature_local() as evaluated in ptycho256.hip.
+// params_signature_local() as evaluated in ptycho256.hip and in ptycho_gen.hip.
 unsigned long long params_signature_256();
+unsigned long long params_signature_gen();
 
 // forward launch modes
 enum : int {
@@ -176,6 +180,16 @@ int launch_chain256_forward(const Params& p, hipStream_t st);   // probe spectra
 int launch_forward256_kernel(const Params& p, hipStream_t st);  // forward kernel only (p.fwd_mode)
 int launch_chain256_adjoint(const Params& p, hipStream_t st);
 int launch_probe_grad256(const Params& p, hipStream_t st);
+// Generic-size chain (ptycho_gen.hip): Nr x Nc patterns, 2..512 per axis, prime factors 2, 3, 5, 7.
+bool gen_supported(int nr, int nc);
+int launch_gen_forward(const Params& p, hipStream_t st);         // probe spectra + forward kernel
+int launch_gen_forward_kernel(const Params& p, hipStream_t st);  // forward kernel only (p.fwd_mode)
+int launch_gen_loss(const Params& p, int ntiles, hipStream_t st);  // per-block partials into p.loss_part
+int gen_loss_parts(int ntiles);                                  // number of those partials
+int launch_gen_adjoint(const Params& p, hipStream_t st);
+int launch_gen_obj_reduce(const Params& p, hipStream_t st);
+int launch_gen_probe_grad(const Params& p, hipStream_t st);      // batch reduce + inverse transform
+int launch_gen_prop_reduce(const Params& p, cf* grad_prop, hipStream_t st);
 // Amplitude centring of measured 256 x 256 patterns (preprocess.hip), `blocks` workgroups with one
 // 256 x 256 complex64 wave each in ws.
 int launch_center256(int P, int blocks, const float* I, const float* com, float* out, float* sums, cf* ws,

@@ -60,13 +60,16 @@ def _need(t: torch.Tensor, name: str, dtype: torch.dtype, shape: tuple, device: 
 
 
 def _desc(S, M, B, N, H, W, num_gpts, loss_type, mean_intensity, flags, segment_slices=0) -> nat.QemDesc:
+    """Descriptor for patterns of shape N = n or (Nr, Nc)."""
     if loss_type not in nat.LOSS_TYPES:
         raise ValueError(f"Unknown loss type {loss_type}, should be one of {sorted(nat.LOSS_TYPES)}")
+    nr, nc = (N, N) if isinstance(N, int) else (int(N[0]), int(N[1]))
     return nat.QemDesc(
         num_slices=S,
         num_modes=M,
         batch=B,
-        roi=N,
+        roi=nr,
+        roi_cols=nc,
         obj_rows=H,
         obj_cols=W,
         num_gpts=int(num_gpts),
@@ -87,13 +90,13 @@ def workspace_bytes(desc: nat.QemDesc) -> int:
 _ONES_MASKS: dict = {}
 
 
-def _ones_mask(n: int, device: torch.device) -> torch.Tensor:
+def _ones_mask(shape: tuple, device: torch.device) -> torch.Tensor:
     """The default detector mask (all ones, ``dataset_models.py:177-187``), one read-only tensor per
-    (size, device), so that a batch without a mask costs no fill kernel."""
-    key = (n, device.type, device.index)
+    (shape, device), so that a batch without a mask costs no fill kernel."""
+    key = (tuple(shape), device.type, device.index)
     m = _ONES_MASKS.get(key)
     if m is None:
-        m = torch.ones((n, n), dtype=torch.float32, device=device)
+        m = torch.ones(tuple(shape), dtype=torch.float32, device=device)
         _ONES_MASKS[key] = m
     return m
 
@@ -118,30 +121,30 @@ class MultisliceLoss(torch.autograd.Function):
         if obj.dim() != 3:
             raise ValueError(f"obj must have shape (S, H, W), got {tuple(obj.shape)}")
         if probe.dim() != 3:
-            raise ValueError(f"probe must have shape (M, N, N), got {tuple(probe.shape)}")
+            raise ValueError(f"probe must have shape (M, Nr, Nc), got {tuple(probe.shape)}")
         S, H, W = obj.shape
-        M, N, N2 = probe.shape
-        if N != N2:
-            raise ValueError(f"probe must be square, got {tuple(probe.shape)}")
-        if N not in nat.SUPPORTED_ROI:
-            raise ValueError(f"roi {N} not supported, expected one of {nat.SUPPORTED_ROI}")
+        M, NR, NC = probe.shape
+        if not nat.roi_supported(NR, NC):
+            raise ValueError(f"roi_shape ({NR}, {NC}) not supported: square {nat.SUPPORTED_ROI}, or "
+                             f"2..{nat.GENERIC_ROI_MAX} per axis with prime factors 2, 3, 5 and 7 only")
+        roi = (NR, NC)
         if positions.dim() != 2:
             raise ValueError(f"positions must have shape (B, 2), got {tuple(positions.shape)}")
         B = positions.shape[0]
         if B < 1:
             raise ValueError("batch must hold at least one scan position")
         obj_c = _need(obj, "obj", torch.complex64, (S, H, W), device)
-        probe_c = _need(probe, "probe", torch.complex64, (M, N, N), device)
+        probe_c = _need(probe, "probe", torch.complex64, (M, NR, NC), device)
         pos_c = _need(positions, "positions", torch.float32, (B, 2), device)
         if S > 1:
-            prop_c = _need(propagators, "propagators", torch.complex64, (S - 1, N, N), device)
+            prop_c = _need(propagators, "propagators", torch.complex64, (S - 1, NR, NC), device)
         else:
             prop_c = None
-        tgt_c = _need(targets, "targets", torch.float32, (B, N, N), device)
+        tgt_c = _need(targets, "targets", torch.float32, (B, NR, NC), device)
         if mask is None:
-            mask_c = _ones_mask(N, device)
+            mask_c = _ones_mask(roi, device)
         else:
-            mask_c = _need(mask, "mask", torch.float32, (N, N), device)
+            mask_c = _need(mask, "mask", torch.float32, roi, device)
         desc_c = None if descan is None else _need(descan, "descan", torch.float32, (B, 2), device)
 
         # grad mode is always off inside Function.forward: the caller's mode arrives as grad_enabled
@@ -156,10 +159,10 @@ class MultisliceLoss(torch.autograd.Function):
                 flags |= nat.QEM_FLAG_GRAD_DESCAN
             if grad_prop:
                 flags |= nat.QEM_FLAG_GRAD_PROPAGATORS
-        d = _desc(S, M, B, N, H, W, num_gpts, loss_type, mean_intensity, flags, segment_slices)
+        d = _desc(S, M, B, roi, H, W, num_gpts, loss_type, mean_intensity, flags, segment_slices)
         ws = torch.empty(workspace_bytes(d), dtype=torch.uint8, device=device)
         loss = torch.empty((), dtype=torch.float32, device=device)
-        intens = torch.empty((B, N, N), dtype=torch.float32, device=device) if return_intensities else None
+        intens = torch.empty((B, NR, NC), dtype=torch.float32, device=device) if return_intensities else None
         lib = nat.load()
         rc = lib.qem_ptycho_forward_loss(
             ctypes.byref(d), _ptr(obj_c), _ptr(probe_c), _ptr(pos_c), _ptr(prop_c), _ptr(tgt_c), _ptr(mask_c),
@@ -228,16 +231,18 @@ def multislice_loss(
     Args:
         obj: (S, H, W) complex64 transmission after hard constraints (``obj_model.obj``; a real
             potential V must be passed as ``torch.exp(1j * V)``, ``object_models.py:253-254``).
-        probe: (M, N, N) complex64 probe modes (``probe_model.probe``), corner-centred.
+        probe: (M, Nr, Nc) complex64 probe modes (``probe_model.probe``), corner-centred; (Nr, Nc) is the
+            dataset's roi_shape: square 32, 64, 128 or 256 (fixed-size kernels) or any 2..512 per axis
+            with prime factors 2, 3, 5, 7 (generic-size kernels).
         positions: (B, 2) float32 scan positions in pixels (``dset.scan_positions_px[batch]``).
-        propagators: (S-1, N, N) complex64 (``ptycho.propagators``), or None for one slice.
-        targets: (B, N, N) float32 fftshifted targets (``dset.targets[batch]``).
-        mask: (N, N) float32 detector mask (``dset.detector_mask``), default ones.
+        propagators: (S-1, Nr, Nc) complex64 (``ptycho.propagators``), or None for one slice.
+        targets: (B, Nr, Nc) float32 fftshifted targets (``dset.targets[batch]``).
+        mask: (Nr, Nc) float32 detector mask (``dset.detector_mask``), default ones.
         descan: (B, 2) float32 descan shifts or None (``dataset_models.py:1495-1498``).
         num_gpts: total number of scan positions of the dataset.
         mean_intensity: ``dset.mean_diffraction_intensity``.
         loss_type: one of l2_amplitude, l1_amplitude, l2_intensity, l1_intensity, poisson.
-        return_intensities: also return the (B, N, N) fftshifted predicted intensities.
+        return_intensities: also return the (B, Nr, Nc) fftshifted predicted intensities.
         segment_slices: 0 keeps every slice's incident wave for the backward pass; k in [2, S) keeps
             only every k-th one and recomputes the others in the backward pass, one k-slice segment
             at a time (less memory, one extra propagation per slice).
@@ -262,10 +267,9 @@ def forward_intensities(
     propagators: Optional[torch.Tensor],
     descan: Optional[torch.Tensor] = None,
 ) -> torch.Tensor:
-    """Predicted fftshifted intensities (B, N, N) of the forward model (no gradient)."""
+    """Predicted fftshifted intensities (B, Nr, Nc) of the forward model (no gradient)."""
     B = positions.shape[0]
-    N = probe.shape[-1]
-    tgt = torch.zeros((B, N, N), dtype=torch.float32, device=obj.device)
+    tgt = torch.zeros((B,) + tuple(probe.shape[-2:]), dtype=torch.float32, device=obj.device)
     with torch.no_grad():
         _, intens = multislice_loss(obj, probe, positions, propagators, tgt, None, descan, num_gpts=B,
                                     mean_intensity=1.0, return_intensities=True)
@@ -359,7 +363,7 @@ def analytical_gradients(
     pos_c = _need(positions.detach(), "positions", torch.float32, (B, 2), device)
     prop_c = _need(propagators, "propagators", torch.complex64, (S - 1, N, N), device) if S > 1 else None
     tgt_c = _need(targets, "targets", torch.float32, (B, N, N), device)
-    mask_c = _ones_mask(N, device) if mask is None else \
+    mask_c = _ones_mask((N, N), device) if mask is None else \
         _need(mask, "mask", torch.float32, (N, N), device)
     desc_c = None if descan is None else _need(descan.detach(), "descan", torch.float32, (B, 2), device)
     d = _desc(S, M, B, N, H, W, num_gpts, loss_type, mean_intensity, nat.QEM_FLAG_STORE_WAVES)

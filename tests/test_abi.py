@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
     lib = nat.load()
     for name in declared_functions():
         assert hasattr(lib, name), name
-    assert lib.qem_abi_version() == 102
+    assert lib.qem_abi_version() == 103
 
 
 def _desc(**kw):
@@ -67,7 +67,9 @@ def test_checkpoint_workspace_formula():
 
 @pytest.mark.parametrize("field,value,code", [
     ("segment_slices", -1, nat.QEM_ERR_ARG),
-    ("roi", 96, nat.QEM_ERR_UNSUPPORTED), ("num_slices", 0, nat.QEM_ERR_ARG), ("batch", 0, nat.QEM_ERR_ARG),
+    ("roi", 97, nat.QEM_ERR_UNSUPPORTED), ("roi", 1024, nat.QEM_ERR_UNSUPPORTED), ("roi", 1, nat.QEM_ERR_UNSUPPORTED),
+    ("roi_cols", 11, nat.QEM_ERR_UNSUPPORTED), ("roi_cols", -1, nat.QEM_ERR_ARG), ("obj_cols", 120, nat.QEM_ERR_ARG),
+    ("num_slices", 0, nat.QEM_ERR_ARG), ("batch", 0, nat.QEM_ERR_ARG),
     ("obj_rows", 64, nat.QEM_ERR_ARG), ("loss_type", 9, nat.QEM_ERR_ARG), ("mean_intensity", 0.0, nat.QEM_ERR_ARG),
     ("num_gpts", 0, nat.QEM_ERR_ARG), ("num_modes", 0, nat.QEM_ERR_ARG),
     ("batch", 32768, nat.QEM_ERR_ARG),  # 32768 x 4 modes x 128^2 = 2^31 far-field elements
@@ -117,3 +119,19 @@ def test_python_boundary_rejects_cpu_tensors():
     with pytest.raises(ValueError):
         qp.multislice_loss(obj, probe, torch.zeros((1, 2)), None, torch.zeros((1, 32, 32)), num_gpts=1,
                            mean_intensity=1.0)
+
+
+def test_generic_shapes_are_accepted_and_sized():
+    """Nr x Nc patterns with prime factors 2, 3, 5, 7 run the generic kernels: stored waves, far field and
+    the per-item row-transformed wave are Nr * Nc complex64 each, the shift ramps Nr + Nc per pattern."""
+    for nr, nc in ((96, 96), (160, 192), (48, 80), (100, 120), (16, 16), (512, 7), (2, 2)):
+        assert nat.roi_supported(nr, nc)
+        d = _desc(roi=nr, roi_cols=nc, obj_rows=600, obj_cols=600)
+        ws = qp.workspace_bytes(d)
+        per = 256 * 4 * nr * nc * 8
+        assert ws >= per * (8 + 1 + 1)  # 8 stored slices, far field, row-transformed wave
+        assert ws - qp.workspace_bytes(_desc(roi=nr, roi_cols=nc, obj_rows=600, obj_cols=600, flags=0)) == 8 * per
+    for nr, nc in ((97, 96), (96, 11), (1024, 64), (1, 8)):
+        assert not nat.roi_supported(nr, nc)
+    # roi_cols 0 is the square pattern
+    assert qp.workspace_bytes(_desc(roi_cols=0)) == qp.workspace_bytes(_desc(roi_cols=128))
