@@ -42,9 +42,10 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def fft_flops(n: int) -> float:
-    """Textbook complex 2-D FFT flop count 5 N^2 log2(N^2)."""
-    return 5.0 * n * n * np.log2(n * n)
+def fft_flops(n) -> float:
+    """Textbook complex 2-D FFT flop count 5 P log2(P) for P = Nr * Nc pixels (n = N or (Nr, Nc))."""
+    nr, nc = synth._shape2(n)
+    return 5.0 * nr * nc * np.log2(nr * nc)
 
 
 def make_inputs(spec: synth.ProblemSpec, dev: torch.device, seed: int = 0) -> dict:
@@ -56,7 +57,7 @@ def make_inputs(spec: synth.ProblemSpec, dev: torch.device, seed: int = 0) -> di
     P = pos.shape[0]
     gen = torch.Generator(device=dev)
     gen.manual_seed(seed + 1)
-    amps = torch.empty((P, spec.roi, spec.roi), dtype=torch.float32, device=dev)
+    amps = torch.empty((P,) + synth._shape2(spec.roi), dtype=torch.float32, device=dev)
     total = 0.0
     for i in range(0, P, spec.batch):
         I = qp.forward_intensities(obj_true, probe, pos[i:i + spec.batch], props)
@@ -363,7 +364,8 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "complex64",
         "data": "synthetic",
-        "config": {"workload": spec.name, "slices": S, "object": [H, W], "roi": N, "modes": M,
+        "config": {"workload": spec.name, "slices": S, "object": [H, W], "roi": N if isinstance(N, int) else list(N),
+                   "modes": M,
                    "scan": list(spec.scan), "patterns": P, "batch": B, "parallelism": "single-gpu",
                    "segment_slices": args.segment},
         "peak_hbm_bytes": int(peak_hbm),

@@ -11,7 +11,7 @@
 //   column pass  chunks of columns: column DFT -> 2-D spectrum -> propagator (far field, probe-gradient
 //                epilogue) -> inverse column DFT -> W
 //
-// A chunk is staged in LDS and transformed there by a Stockham autosort FFT (mixed radix 4, 2, 3, 5, 7,
+// A chunk is staged in LDS and transformed there by a Stockham autosort FFT (mixed radix 8, 4, 2, 3, 5, 7,
 // ping-pong between two LDS buffers, twiddles from a per-length table built in double precision).  A
 // butterfly of radix R at a stage with stride s reads a_j = x[w + j n/R] (w = p s + q) and writes
 // y[q + s (R p + k)] = W_n^{p s k} sum_j a_j W_R^{j k}, the decimation-in-frequency form that leaves the
@@ -41,20 +41,21 @@ struct Smem {
   cf tw_c[NMAX];    // exp(-2 pi i k / Nc): row transforms
   float fq_r[NMAX]; // fftfreq(Nr)[k] in float32
   float fq_c[NMAX]; // fftfreq(Nc)[k] in float32
-  cf aux[NMAX];     // per-item column factors (descan)
+  cf aux[NMAX];     // per-item column factors exp(-2 pi i fftfreq(Nc)[c] d_c) of the descan ramp
+  cf aux_r[NMAX];   // per-item row factors exp(-2 pi i fftfreq(Nr)[r] d_r) of the descan ramp
   double red[T / 64];
 };
 
-// Radices of a length-n transform, 4 bits per stage, in stage order (4s first, then 2, 3, 5, 7).
+// Radices of a length-n transform, 4 bits per stage, in stage order (8s first, then 4, 2, 3, 5, 7).
 struct Axis {
   unsigned long long code;
   int ns;
 };
 __device__ __forceinline__ Axis make_axis(int n) {
   Axis a{0ull, 0};
-  const int rads[5] = {4, 2, 3, 5, 7};
+  const int rads[6] = {8, 4, 2, 3, 5, 7};
 #pragma unroll
-  for (int i = 0; i < 5; ++i) {
+  for (int i = 0; i < 6; ++i) {
     const int r = rads[i];
     while (n % r == 0 && n > 1 && a.ns < MAXST) {
       a.code |= (unsigned long long)r << (4 * a.ns);
@@ -101,49 +102,117 @@ __device__ __forceinline__ Geo make_geo(const Params& p) {
   return g;
 }
 
-// Length-R DFT of a (unnormalised; conjugate roots if INV); wr[i] = W_R^i (conjugated if INV).
+// 4-point DFT (unnormalised; conjugate roots if INV).
+template <bool INV>
+__device__ __forceinline__ void dft4(cf a0, cf a1, cf a2, cf a3, cf& o0, cf& o1, cf& o2, cf& o3) {
+  const cf t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), d = csub(a1, a3);
+  const cf t3 = INV ? mk(-d.y, d.x) : mk(d.y, -d.x);  // +i d (inverse) or -i d (forward)
+  o0 = cadd(t0, t2);
+  o2 = csub(t0, t2);
+  o1 = cadd(t1, t3);
+  o3 = csub(t1, t3);
+}
+
+// cos and sin of 2 pi m / R for the odd radices (m = 1 .. (R - 1) / 2).
+template <int R>
+struct OddRoots;
+template <>
+struct OddRoots<3> {
+  static constexpr float c[2] = {1.0f, -0.5f};
+  static constexpr float s[2] = {0.0f, 0.866025403784438647f};
+};
+template <>
+struct OddRoots<5> {
+  static constexpr float c[3] = {1.0f, 0.309016994374947424f, -0.809016994374947424f};
+  static constexpr float s[3] = {0.0f, 0.951056516295153572f, 0.587785252292473129f};
+};
+template <>
+struct OddRoots<7> {
+  static constexpr float c[4] = {1.0f, 0.623489801858733531f, -0.222520933956314404f, -0.900968867902419126f};
+  static constexpr float s[4] = {0.0f, 0.781831482468029809f, 0.974927912181823607f, 0.433883739117558120f};
+};
+
+// Length-R DFT of a (unnormalised; conjugate roots if INV).  Odd R: with s_j = a_j + a_{R-j} and
+// d_j = a_j - a_{R-j} (j = 1 .. h, h = (R - 1) / 2),
+//   X_k = a_0 + sum_j s_j cos(2 pi j k / R) -+ i sum_j d_j sin(2 pi j k / R),  X_{R-k} the same with +-,
+// the upper sign for the forward transform.
 template <int R, bool INV>
-__device__ __forceinline__ void dft(const cf (&a)[R], cf (&o)[R], const cf (&wr)[R]) {
+__device__ __forceinline__ void dft(const cf (&a)[R], cf (&o)[R]) {
   if constexpr (R == 2) {
     o[0] = cadd(a[0], a[1]);
     o[1] = csub(a[0], a[1]);
   } else if constexpr (R == 4) {
-    const cf t0 = cadd(a[0], a[2]), t1 = csub(a[0], a[2]), t2 = cadd(a[1], a[3]), d = csub(a[1], a[3]);
-    const cf t3 = INV ? mk(-d.y, d.x) : mk(d.y, -d.x);  // +i d (inverse) or -i d (forward)
-    o[0] = cadd(t0, t2);
-    o[2] = csub(t0, t2);
-    o[1] = cadd(t1, t3);
-    o[3] = csub(t1, t3);
+    dft4<INV>(a[0], a[1], a[2], a[3], o[0], o[1], o[2], o[3]);
+  } else if constexpr (R == 8) {
+    // two 4-point DFTs of the even and odd inputs, then X_k = E_k + W8^k O_k, X_{k+4} = E_k - W8^k O_k
+    cf e0, e1, e2, e3, q0, q1, q2, q3;
+    dft4<INV>(a[0], a[2], a[4], a[6], e0, e1, e2, e3);
+    dft4<INV>(a[1], a[3], a[5], a[7], q0, q1, q2, q3);
+    constexpr float h = 0.70710678118654752f;
+    const cf w1 = INV ? mk(h * (q1.x - q1.y), h * (q1.x + q1.y)) : mk(h * (q1.x + q1.y), h * (q1.y - q1.x));
+    const cf w2 = INV ? mk(-q2.y, q2.x) : mk(q2.y, -q2.x);
+    const cf w3 = INV ? mk(-h * (q3.x + q3.y), h * (q3.x - q3.y)) : mk(h * (q3.y - q3.x), -h * (q3.x + q3.y));
+    o[0] = cadd(e0, q0);
+    o[4] = csub(e0, q0);
+    o[1] = cadd(e1, w1);
+    o[5] = csub(e1, w1);
+    o[2] = cadd(e2, w2);
+    o[6] = csub(e2, w2);
+    o[3] = cadd(e3, w3);
+    o[7] = csub(e3, w3);
   } else {
+    constexpr int h = (R - 1) / 2;
+    cf sj[h + 1], dj[h + 1];
+    cf dc = a[0];
 #pragma unroll
-    for (int k = 0; k < R; ++k) {
-      cf acc = a[0];
+    for (int j = 1; j <= h; ++j) {
+      sj[j] = cadd(a[j], a[R - j]);
+      dj[j] = csub(a[j], a[R - j]);
+      dc = cadd(dc, sj[j]);
+    }
+    o[0] = dc;
 #pragma unroll
-      for (int j = 1; j < R; ++j) acc = cadd(acc, cmul(a[j], wr[(j * k) % R]));
-      o[k] = acc;
+    for (int k = 1; k <= h; ++k) {
+      cf re = a[0], im = mk(0.0f, 0.0f);
+#pragma unroll
+      for (int j = 1; j <= h; ++j) {
+        int m = (j * k) % R;
+        const float sg = m <= h ? 1.0f : -1.0f;  // sin(2 pi (R - m) / R) = -sin(2 pi m / R)
+        m = m <= h ? m : R - m;
+        re = mk(re.x + OddRoots<R>::c[m] * sj[j].x, re.y + OddRoots<R>::c[m] * sj[j].y);
+        im = mk(im.x + sg * OddRoots<R>::s[m] * dj[j].x, im.y + sg * OddRoots<R>::s[m] * dj[j].y);
+      }
+      // forward: X_k = re - i im, X_{R-k} = re + i im; inverse: the other way round
+      const cf mi = mk(im.y, -im.x);  // -i im
+      o[k] = INV ? csub(re, mi) : cadd(re, mi);
+      o[R - k] = INV ? cadd(re, mi) : csub(re, mi);
     }
   }
 }
+
+// Exact quotient x / d for 0 <= x < 2^22 and 1 <= d < 2^12 by a float reciprocal: (x + 1/2) / d lies at
+// least 1/(2d) from an integer and the two roundings move it by less than (x + 1/2) 2^-23 / d.
+struct Div {
+  float r;
+  __device__ __forceinline__ explicit Div(int d) : r(1.0f / (float)d) {}
+  __device__ __forceinline__ int operator()(int x) const { return (int)(((float)x + 0.5f) * r); }
+};
 
 // One Stockham stage of radix R over lc lines of length n (line stride LS) from x to y.
 template <int R, bool INV>
 __device__ __forceinline__ void stage(const cf* x, cf* y, int lc, int n, int LS, int s, const cf* tw) {
   const int nq = n / R;  // butterflies per line
-  cf wr[R];
-#pragma unroll
-  for (int i = 0; i < R; ++i) {
-    const cf w = tw[i * nq];
-    wr[i] = INV ? mk(w.x, -w.y) : w;
-  }
   const int nb = lc * nq;
+  const Div dq(nq), dsd(s);
   for (int bi = threadIdx.x; bi < nb; bi += T) {
-    const int l = bi / nq, w = bi - l * nq;
-    const int q = w % s, ps = w - q;  // w = p s + q
+    const int l = dq(bi), w = bi - l * nq;
+    const int p = dsd(w), q = w - p * s;
+    const int ps = p * s;
     const cf* xi = x + l * LS + w;
     cf a[R], o[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) a[j] = xi[j * nq];
-    dft<R, INV>(a, o, wr);
+    dft<R, INV>(a, o);
     cf* yo = y + l * LS + q + ps * R;
     yo[0] = o[0];
 #pragma unroll
@@ -163,6 +232,7 @@ __device__ cf* lds_fft(cf* x, cf* y, int lc, int n, const Axis& ax, const cf* tw
   for (int st = 0; st < ax.ns; ++st) {
     const int r = (int)((ax.code >> (4 * st)) & 15ull);
     switch (r) {
+      case 8: stage<8, INV>(x, y, lc, n, LS, s, tw); break;
       case 4: stage<4, INV>(x, y, lc, n, LS, s, tw); break;
       case 2: stage<2, INV>(x, y, lc, n, LS, s, tw); break;
       case 3: stage<3, INV>(x, y, lc, n, LS, s, tw); break;
@@ -180,15 +250,37 @@ __device__ cf* lds_fft(cf* x, cf* y, int lc, int n, const Axis& ax, const cf* tw
 
 enum : int { NONE = 0, DFT_FWD = 1, DFT_INV = 2 };
 
-struct NoOp {
-  __device__ __forceinline__ cf operator()(int, int, cf v) const { return v; }
+constexpr int KPT = EB / T;  // chunk elements per thread
+
+// Elementwise work of a pass, in two parts so that every thread issues all of its global loads before
+// it uses any: fetch(r, c) loads what pixel (r, c) needs into an Aux, apply(r, c, v, aux) returns the new
+// value (and may store).
+struct Aux {
+  cf a, b;
 };
+template <typename F, typename A>
+struct Op2 {
+  F fetch;
+  A apply;
+};
+template <typename F, typename A>
+__device__ __forceinline__ Op2<F, A> op2(F f, A a) {
+  return Op2<F, A>{f, a};
+}
+struct NoFetch {
+  __device__ __forceinline__ Aux operator()(int, int) const { return Aux{}; }
+};
+struct Keep {
+  __device__ __forceinline__ cf operator()(int, int, cf v, const Aux&) const { return v; }
+};
+__device__ __forceinline__ Op2<NoFetch, Keep> no_op() { return Op2<NoFetch, Keep>{NoFetch(), Keep()}; }
 
 // A pass over the lines of one item: rows (COLS false, lines of length NC) or columns (lines of length
-// NR), in chunks that fit an LDS buffer.  Each chunk: load src -> PRE transform -> v = op(r, c, v) on
-// every element -> POST transform -> store to dst (if not null).  Chunks touch disjoint lines and read
-// all of theirs before writing, so src may equal dst.  Column chunks map consecutive threads to
-// consecutive columns of a row (coalesced), row chunks to consecutive columns of one row.
+// NR), in chunks that fit an LDS buffer.  Each chunk: load src -> PRE transform -> apply the op to every
+// element -> POST transform -> store to dst (if not null).  Chunks touch disjoint lines and read all of
+// theirs before writing, so src may equal dst.  Element e of a chunk is held by thread e % T; column
+// chunks map consecutive elements to consecutive columns of a row (coalesced), row chunks to
+// consecutive columns of one row.
 template <int PRE, int POST, bool COLS, typename Op>
 __device__ void pass(const cf* src, cf* dst, Smem& sm, const Geo& g, Op&& op) {
   const int n = COLS ? g.NR : g.NC;
@@ -198,54 +290,89 @@ __device__ void pass(const cf* src, cf* dst, Smem& sm, const Geo& g, Op&& op) {
   const Axis& ax = COLS ? g.ar : g.ac;
   const cf* tw = COLS ? sm.tw_r : sm.tw_c;
   const int NC = g.NC;
+  const Div dn(n);
+  // element e = threadIdx.x + k T (k < KPT) of the chunk of lines [l0, l0 + lc): line l, index i
+  auto split = [&](int l0, int lc, const Div& dl, int e, int& l, int& i) {
+    if constexpr (COLS) {
+      i = dl(e);
+      l = e - i * lc;
+    } else {
+      // through the absolute element index: the split then depends on l0, so the compiler cannot
+      // hoist every chunk's (l, i) out of the chunk and slice loops and keep them live in registers
+      const int ea = e + l0 * n;
+      const int la = dn(ea);
+      i = ea - la * n;
+      l = la - l0;
+    }
+  };
+  auto gidx = [&](int l0, int l, int i) { return COLS ? i * NC + l0 + l : (l0 + l) * NC + i; };
+  // the chunk's values are loaded one chunk ahead (before the previous chunk's transforms), and the
+  // op's own global loads before the chunk's first transform: both latencies hide behind LDS work
+  cf v[KPT];
+  {
+    const int lc = min(L, nl);
+    const Div dl(lc);
+#pragma unroll
+    for (int k = 0; k < KPT; ++k) {
+      const int e = threadIdx.x + k * T;
+      int l, i;
+      split(0, lc, dl, e, l, i);
+      if (e < lc * n) v[k] = src[gidx(0, l, i)];
+    }
+  }
   for (int l0 = 0; l0 < nl; l0 += L) {
     const int lc = min(L, nl - l0);
     const int E = lc * n;
+    const Div dl(lc);
     cf* A = sm.buf[0];
     cf* Bf = sm.buf[1];
-    for (int e = threadIdx.x; e < E; e += T) {
+#pragma unroll
+    for (int k = 0; k < KPT; ++k) {
+      const int e = threadIdx.x + k * T;
       int l, i;
-      if constexpr (COLS) {
-        i = e / lc;
-        l = e - i * lc;
-      } else {
-        l = e / n;
-        i = e - l * n;
-      }
-      const size_t gi = COLS ? (size_t)i * NC + l0 + l : (size_t)(l0 + l) * NC + i;
-      A[l * LS + i] = src[gi];
+      split(l0, lc, dl, e, l, i);
+      if (e < E) A[l * LS + i] = v[k];
     }
     __syncthreads();
+    Aux x[KPT];
+#pragma unroll
+    for (int k = 0; k < KPT; ++k) {
+      const int e = threadIdx.x + k * T;
+      int l, i;
+      split(l0, lc, dl, e, l, i);
+      if (e < E) x[k] = op.fetch(COLS ? i : l0 + l, COLS ? l0 + l : i);
+    }
+    if (l0 + L < nl) {  // the next chunk's lines are disjoint from this one's: safe to read before the stores
+      const int l1 = l0 + L, lc1 = min(L, nl - l1);
+      const Div dl1(lc1);
+#pragma unroll
+      for (int k = 0; k < KPT; ++k) {
+        const int e = threadIdx.x + k * T;
+        int l, i;
+        split(l1, lc1, dl1, e, l, i);
+        if (e < lc1 * n) v[k] = src[gidx(l1, l, i)];
+      }
+    }
     cf* R = A;
     if constexpr (PRE != NONE) R = lds_fft<PRE == DFT_INV>(A, Bf, lc, n, ax, tw);
-    for (int e = threadIdx.x; e < E; e += T) {
+#pragma unroll
+    for (int k = 0; k < KPT; ++k) {
+      const int e = threadIdx.x + k * T;
       int l, i;
-      if constexpr (COLS) {
-        i = e / lc;
-        l = e - i * lc;
-      } else {
-        l = e / n;
-        i = e - l * n;
-      }
-      const int r = COLS ? i : l0 + l, c = COLS ? l0 + l : i;
-      R[l * LS + i] = op(r, c, R[l * LS + i]);
+      split(l0, lc, dl, e, l, i);
+      if (e < E) R[l * LS + i] = op.apply(COLS ? i : l0 + l, COLS ? l0 + l : i, R[l * LS + i], x[k]);
     }
     if constexpr (POST != NONE) {
       __syncthreads();
       R = lds_fft<POST == DFT_INV>(R, R == A ? Bf : A, lc, n, ax, tw);
     }
     if (dst != nullptr) {
-      for (int e = threadIdx.x; e < E; e += T) {
+#pragma unroll
+      for (int k = 0; k < KPT; ++k) {
+        const int e = threadIdx.x + k * T;
         int l, i;
-        if constexpr (COLS) {
-          i = e / lc;
-          l = e - i * lc;
-        } else {
-          l = e / n;
-          i = e - l * n;
-        }
-        const size_t gi = COLS ? (size_t)i * NC + l0 + l : (size_t)(l0 + l) * NC + i;
-        dst[gi] = R[l * LS + i];
+        split(l0, lc, dl, e, l, i);
+        if (e < E) dst[gidx(l0, l, i)] = R[l * LS + i];
       }
     }
     __syncthreads();  // the chunk is stored before the next one overwrites the buffers
@@ -264,8 +391,13 @@ __device__ __forceinline__ cf obj_patch(const cf* O, int r0m, int c0m, int r, in
   return O[(size_t)y * W + x];
 }
 
-__device__ __forceinline__ cf ramp2(float fr, float fc, float dr, float dc) {
-  return cmul(expm2pi(fr * dr), expm2pi(fc * dc));
+// The descan ramp's row and column factors of pattern b into sm.aux_r / sm.aux (the ramp at (r, c) is
+// aux_r[r] aux[c], ptychography_base.py:903-905 with ptycho_utils.py:156-176).
+__device__ __forceinline__ void descan_tables(Smem& sm, const Geo& g, const float* descan, int b) {
+  const float dr = descan[2 * b], dc = descan[2 * b + 1];
+  for (int k = threadIdx.x; k < g.NR; k += T) sm.aux_r[k] = expm2pi(sm.fq_r[k] * dr);
+  for (int k = threadIdx.x; k < g.NC; k += T) sm.aux[k] = expm2pi(sm.fq_c[k] * dc);
+  __syncthreads();
 }
 
 // pk[m] = fft2(probe[m]); the blocks past the modes fill the shift-ramp table
@@ -288,8 +420,8 @@ __global__ __launch_bounds__(T) void k_spectrum_gen(Params p) {
   const size_t N2 = (size_t)NR * NC;
   const int m = blockIdx.x;
   cf* Wb = p.scratch + (size_t)m * N2;
-  pass<NONE, DFT_FWD, false>(p.probe + (size_t)m * N2, Wb, sm, g, NoOp());
-  pass<DFT_FWD, NONE, true>(Wb, p.pk + (size_t)m * N2, sm, g, NoOp());
+  pass<NONE, DFT_FWD, false>(p.probe + (size_t)m * N2, Wb, sm, g, no_op());
+  pass<DFT_FWD, NONE, true>(Wb, p.pk + (size_t)m * N2, sm, g, no_op());
 }
 
 // Forward multislice of one (pattern, mode); p.fwd_mode as in the other chains.  W holds the
@@ -322,25 +454,30 @@ __global__ __launch_bounds__(T) void k_forward_gen(Params p) {
   if (far) {
     // psi_0 = ifft2(fft2(P_m) ramp_b): inverse column DFT here, inverse row DFT in the first row pass
     const cf* rr = p.ramps + (size_t)b * (NR + NC);
-    pass<NONE, DFT_INV, true>(p.pk + (size_t)m * N2, Wb, sm, g, [&](int r, int c, cf v) {
-      return cscale(cmul(v, cmul(rr[r], rr[NR + c])), inv_n2);
-    });
+    pass<NONE, DFT_INV, true>(p.pk + (size_t)m * N2, Wb, sm, g,
+                              op2([&](int r, int c) { return Aux{rr[r], rr[NR + c]}; },
+                                  [&](int, int, cf v, const Aux& x) { return cscale(cmul(v, cmul(x.a, x.b)), inv_n2); }));
   }
+  if (DESCAN && far) descan_tables(sm, g, p.descan, b);
   for (int s = p.s0; s < p.s1; ++s) {
     const cf* O = p.obj + (size_t)s * HW;
     cf* keep = slot(s);
     const bool transmit = far || s + 1 < p.s1;  // a recomputed segment needs only its incident waves
     const bool exit_ramp = DESCAN && far && s == p.S - 1;
-    auto op = [&](int r, int c, cf v) {
-      cf o = mk(0.0f, 0.0f);
-      if (transmit) o = obj_patch(O, r0m, c0m, r, c, g, p.H, p.W);
-      if (keep != nullptr) st_stream(keep + (size_t)r * NC + c, v);
-      if (transmit) {
-        v = cmul(o, v);
-        if (exit_ramp) v = cmul(v, ramp2(sm.fq_r[r], sm.fq_c[c], p.descan[2 * b], p.descan[2 * b + 1]));
-      }
-      return v;
-    };
+    auto op = op2(
+        [&](int r, int c) {
+          Aux x;
+          if (transmit) x.a = obj_patch(O, r0m, c0m, r, c, g, p.H, p.W);
+          return x;
+        },
+        [&](int r, int c, cf v, const Aux& x) {
+          if (keep != nullptr) st_stream(keep + (size_t)r * NC + c, v);
+          if (transmit) {
+            v = cmul(x.a, v);
+            if (exit_ramp) v = cmul(v, cmul(sm.aux_r[r], sm.aux[c]));
+          }
+          return v;
+        });
     if (mode == FWD_RECOMPUTE && s == p.s0) {
       // a segment of an earlier forward starts from its checkpoint wave (natural order)
       const cf* ck = p.ckpt + ((size_t)item * p.nck + p.s0 / p.ck) * N2;
@@ -356,14 +493,16 @@ __global__ __launch_bounds__(T) void k_forward_gen(Params p) {
     if (s + 1 < p.s1) {
       const cf* pr = p.prop + (size_t)s * N2;
       cf* xs = PG ? p.pspec + ((size_t)item * (p.S - 1) + s) * N2 : nullptr;
-      pass<DFT_FWD, DFT_INV, true>(Wb, Wb, sm, g, [&](int r, int c, cf v) {
-        const size_t k = (size_t)r * NC + c;
-        if constexpr (PG) st_stream(xs + k, v);  // fft2 of the transmitted wave, for the propagator gradient
-        return cscale(cmul(v, pr[k]), inv_n2);
-      });
+      pass<DFT_FWD, DFT_INV, true>(Wb, Wb, sm, g,
+                                   op2([&](int r, int c) { return Aux{pr[(size_t)r * NC + c], cf{}}; },
+                                       [&](int r, int c, cf v, const Aux& x) {
+                                         // fft2 of the transmitted wave, for the propagator gradient
+                                         if constexpr (PG) st_stream(xs + (size_t)r * NC + c, v);
+                                         return cscale(cmul(v, x.a), inv_n2);
+                                       }));
     } else if (far) {
       pass<DFT_FWD, NONE, true>(Wb, p.X + (size_t)item * N2, sm, g,
-                                [&](int, int, cf v) { return cscale(v, inv_sqrt); });
+                                op2(NoFetch(), [&](int, int, cf v, const Aux&) { return cscale(v, inv_sqrt); }));
     }
   }
 }
@@ -445,7 +584,9 @@ __global__ __launch_bounds__(T) void k_adjoint_gen(Params p) {
     // g_x = F_ortho^H (2 dL/dI X): inverse column DFT here, inverse row DFT in the first row pass
     const float* gI = p.gI + (size_t)b * N2;
     const float sc = (float)(2.0 * (double)p.dloss[0] / sqrt((double)N2));
-    pass<NONE, DFT_INV, true>(Xs, Wb, sm, g, [&](int r, int c, cf v) { return cscale(v, sc * gI[(size_t)r * NC + c]); });
+    pass<NONE, DFT_INV, true>(Xs, Wb, sm, g,
+                              op2([&](int r, int c) { return Aux{mk(gI[(size_t)r * NC + c], 0.0f), cf{}}; },
+                                  [&](int, int, cf v, const Aux& x) { return cscale(v, sc * x.a.x); }));
   }
 
   double dar = 0.0, dac = 0.0;
@@ -454,45 +595,50 @@ __global__ __launch_bounds__(T) void k_adjoint_gen(Params p) {
     // 2 pi sum f Im(conj(g) x_out) with x_out = ramp O_{S-1} psi_{S-1}, and g <- conj(ramp) g
     const cf* O = p.obj + (size_t)(p.S - 1) * HW;
     const cf* A = p.psi + ((size_t)item * p.wslots + (p.S - 1 - p.s0)) * N2;
-    const float ddr = p.descan[2 * b], ddc = p.descan[2 * b + 1];
-    for (int c = threadIdx.x; c < NC; c += T) sm.aux[c] = expm2pi(sm.fq_c[c] * ddc);
-    __syncthreads();
+    descan_tables(sm, g, p.descan, b);
     const float inv_nc = (float)(1.0 / (double)NC);
-    pass<DFT_INV, DFT_FWD, false>(Wb, Wb, sm, g, [&](int r, int c, cf v) {
-      const cf d = cmul(expm2pi(sm.fq_r[r] * ddr), sm.aux[c]);
-      const cf xo = cmul(cmul(obj_patch(O, r0m, c0m, r, c, g, p.H, p.W), A[(size_t)r * NC + c]), d);
+    pass<DFT_INV, DFT_FWD, false>(Wb, Wb, sm, g, op2([&](int r, int c) {
+      return Aux{obj_patch(O, r0m, c0m, r, c, g, p.H, p.W), A[(size_t)r * NC + c]};
+    }, [&](int r, int c, cf v, const Aux& x) {
+      const cf d = cmul(sm.aux_r[r], sm.aux[c]);
+      const cf xo = cmul(cmul(x.a, x.b), d);
       const float im = cconjmul(v, xo).y;
       dar += (double)(sm.fq_r[r] * im);
       dac += (double)(sm.fq_c[c] * im);
       // 1 / Nc: the forward row DFT is not followed by a column pass, so it leaves R g / Nc, the form
       // every row pass expects (R^H R = Nc for the unnormalised row DFT R)
       return cscale(cmulc(v, d), inv_nc);
-    });
+    }));
   }
   for (int s = p.s1 - 1; s >= p.s0; --s) {
     const cf* O = p.obj + (size_t)s * HW;
     cf* A = p.psi + ((size_t)item * p.wslots + (s - p.s0)) * N2;
     // natural-order g_x of slice s: object-gradient patch conj(psi_s) g over psi_s, g_psi = conj(O_s) g
     pass<DFT_INV, DFT_FWD, false>((!top && s == p.s1 - 1) ? (const cf*)Xs : (const cf*)Wb, Wb, sm, g,
-                                  [&](int r, int c, cf v) {
-                                    const cf o = obj_patch(O, r0m, c0m, r, c, g, p.H, p.W);
-                                    cf* a = A + (size_t)r * NC + c;
-                                    const cf pv = ld_stream(a);
-                                    st_stream(a, cconjmul(pv, v));
-                                    return cconjmul(o, v);
-                                  });
+                                  op2(
+                                      [&](int r, int c) {
+                                        return Aux{obj_patch(O, r0m, c0m, r, c, g, p.H, p.W),
+                                                   ld_stream(A + (size_t)r * NC + c)};
+                                      },
+                                      [&](int r, int c, cf v, const Aux& x) {
+                                        st_stream(A + (size_t)r * NC + c, cconjmul(x.b, v));
+                                        return cconjmul(x.a, v);
+                                      }));
     if (s > 0) {
       // g_x <- ifft2(fft2(g_psi) conj(prop_{s-1})); at a segment bottom it goes to X for the segment below
       const cf* pr = p.prop + (size_t)(s - 1) * N2;
       cf* xs = PG ? p.pspec + ((size_t)item * (p.S - 1) + (s - 1)) * N2 : nullptr;
-      pass<DFT_FWD, DFT_INV, true>(Wb, s == p.s0 ? Xs : Wb, sm, g, [&](int r, int c, cf v) {
+      pass<DFT_FWD, DFT_INV, true>(Wb, s == p.s0 ? Xs : Wb, sm, g, op2([&](int r, int c) {
         const size_t k = (size_t)r * NC + c;
-        if constexpr (PG) {  // this item's dL/dP_{s-1}: conj(fft2(x_{s-1})) fft2(g_psi_s) / (Nr Nc)
-          const cf xq = ld_stream(xs + k);
-          st_stream(xs + k, cscale(cconjmul(xq, v), inv_n2));
-        }
-        return cscale(cmulc(v, pr[k]), inv_n2);
-      });
+        Aux x;
+        x.a = pr[k];
+        if constexpr (PG) x.b = ld_stream(xs + k);
+        return x;
+      }, [&](int r, int c, cf v, const Aux& x) {
+        // this item's dL/dP_{s-1}: conj(fft2(x_{s-1})) fft2(g_psi_s) / (Nr Nc)
+        if constexpr (PG) st_stream(xs + (size_t)r * NC + c, cscale(cconjmul(x.b, v), inv_n2));
+        return cscale(cmulc(v, x.a), inv_n2);
+      }));
     }
   }
   if constexpr (DESCAN) {
@@ -511,13 +657,14 @@ __global__ __launch_bounds__(T) void k_adjoint_gen(Params p) {
   const cf* pk = p.pk + (size_t)m * N2;
   const cf* rr = p.ramps + (size_t)b * (NR + NC);
   double ar = 0.0, ac = 0.0;
-  pass<DFT_FWD, NONE, true>(Wb, Xs, sm, g, [&](int r, int c, cf v) {
-    const cf R = cmul(rr[r], rr[NR + c]);
-    const float im = cconjmul(v, cmul(pk[(size_t)r * NC + c], R)).y;
+  pass<DFT_FWD, NONE, true>(Wb, Xs, sm, g, op2([&](int r, int c) {
+    return Aux{pk[(size_t)r * NC + c], cmul(rr[r], rr[NR + c])};
+  }, [&](int r, int c, cf v, const Aux& x) {
+    const float im = cconjmul(v, cmul(x.a, x.b)).y;
     ar += (double)(sm.fq_r[r] * im);
     ac += (double)(sm.fq_c[c] * im);
-    return cmulc(v, R);
-  });
+    return cmulc(v, x.b);
+  }));
   if (p.flags & QEM_FLAG_GRAD_POSITIONS) {
     const double tr = block_sum<T>(ar, sm.red);
     const double tc = block_sum<T>(ac, sm.red);
@@ -630,9 +777,9 @@ __global__ __launch_bounds__(T) void k_probe_grad_gen(Params p) {
   const int m = blockIdx.x;
   cf* Wb = p.scratch + (size_t)m * N2;
   const float inv_n2 = (float)(1.0 / (double)N2);
-  pass<DFT_INV, NONE, true>(p.gsum + (size_t)m * N2, Wb, sm, g, NoOp());
+  pass<DFT_INV, NONE, true>(p.gsum + (size_t)m * N2, Wb, sm, g, no_op());
   pass<DFT_INV, NONE, false>(Wb, p.grad_probe + (size_t)m * N2, sm, g,
-                             [&](int, int, cf v) { return cscale(v, inv_n2); });
+                             op2(NoFetch(), [&](int, int, cf v, const Aux&) { return cscale(v, inv_n2); }));
 }
 
 // grad_prop[s][k] = sum over the items of their contributions in p.pspec, fixed order.

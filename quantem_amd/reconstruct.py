@@ -648,7 +648,7 @@ def scan_window(spec) -> tuple:
 
 def simulate_amplitudes(spec, prob: dict, device, seed: int = 0) -> tuple:
     """Measured amplitudes on the GPU: the forward kernel's intensities (batches of spec.batch) with
-    Poisson counts at the probe's dose; returns (amplitudes (P, N, N) float32 on device, mean intensity)."""
+    Poisson counts at the probe's dose; returns (amplitudes (P, Nr, Nc) float32 on device, mean intensity)."""
     from .ptychography import forward_intensities
 
     dev = torch.device(device)
@@ -659,7 +659,9 @@ def simulate_amplitudes(spec, prob: dict, device, seed: int = 0) -> tuple:
     gen = torch.Generator(device=dev)
     gen.manual_seed(seed + 1)
     P = pos.shape[0]
-    amps = torch.empty((P, spec.roi, spec.roi), dtype=torch.float32, device=dev)
+    from .synth import _shape2
+
+    amps = torch.empty((P,) + _shape2(spec.roi), dtype=torch.float32, device=dev)
     total = torch.zeros((), dtype=torch.float64, device=dev)
     for i in range(0, P, spec.batch):
         counts = torch.poisson(forward_intensities(obj, probe, pos[i:i + spec.batch], props).clamp_min(0),

@@ -157,7 +157,7 @@ class ProblemSpec:
     name: str = "multislice-8x384-128px-4modes-64x64scan"
     num_slices: int = 8
     obj_shape: tuple = (384, 384)
-    roi: int = 128
+    roi: object = 128  # N (square) or the roi_shape (Nr, Nc)
     num_modes: int = 4
     mode_weights: tuple = (0.7, 0.15, 0.1, 0.05)
     scan: tuple = (64, 64)
@@ -189,6 +189,9 @@ LARGE = ProblemSpec(
     name="multislice-30x1024-256px-8modes-128x128scan",
     num_slices=30, obj_shape=(1024, 1024), roi=256, num_modes=8, mode_weights=(0.86,) + (0.02,) * 7,
     scan=(128, 128), slice_thickness=1.5, n_atoms=2133, batch=512)
+# the headline's slices, modes and scan with 128 x 120 patterns: the generic-size (mixed-radix) kernels
+# on nearly the headline's work per pattern
+GENERIC = ProblemSpec(name="multislice-8x384-128x120px-4modes-64x64scan", roi=(128, 120))
 SINGLE_SLICE = ProblemSpec(
     name="single-slice-192px-64px-32x32scan",
     num_slices=1, obj_shape=(192, 192), roi=64, num_modes=1, mode_weights=(1.0,), scan=(32, 32),
@@ -200,6 +203,7 @@ WORKLOADS = {
     "depth20-256px": DEPTH,
     "large30-1024px": LARGE,
     "single-slice": SINGLE_SLICE,
+    "generic-128x120px": GENERIC,
 }
 
 
@@ -227,8 +231,8 @@ def spec_upsample_factor(spec: "ProblemSpec") -> float:
 def make_problem(spec: ProblemSpec, seed: int = 0, with_data: bool = True) -> dict:
     """Object, probe, positions and propagators (and Poisson data computed by `intensity_fn`).
 
-    Returns numpy arrays: obj (S,H,W) c64, probe (M,N,N) c64, positions (P,2) f32, propagators
-    (S-1,N,N) c64, true_phase (S,H,W) f64.  Data are added by :func:`attach_data`.
+    Returns numpy arrays: obj (S,H,W) c64, probe (M,Nr,Nc) c64, positions (P,2) f32, propagators
+    (S-1,Nr,Nc) c64, true_phase (S,H,W) f64.  Data are added by :func:`attach_data`.
     """
     rng = np.random.default_rng(seed)
     S = spec.num_slices
