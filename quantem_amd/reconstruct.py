@@ -384,7 +384,8 @@ class Reconstruction:
         pp = self.propagator_params
         if pp is None:
             raise ValueError("learning the probe tilt needs propagator_params (sampling, energy, thicknesses)")
-        return fresnel_propagators(int(self.targets.shape[-1]), float(pp["sampling"]), float(pp["energy"]),
+        return fresnel_propagators(tuple(int(x) for x in self.targets.shape[-2:]), float(pp["sampling"]),
+                                   float(pp["energy"]),
                                    pp["thicknesses"], self.probe_tilt)
 
     @property
