@@ -223,7 +223,7 @@ def measure_traffic(timeout_s: float = 150.0) -> dict | None:
         return None
     sf = CALIBRATION_BYTES / (cal_f * 1024.0)
     sw = CALIBRATION_BYTES / (cal_w * 1024.0)
-    kernels = {"adjoint": ("k_adjoint<128",), "forward": ("k_forward<128", "true>"),
+    kernels = {"adjoint": ("k_adjoint<128",), "forward": ("k_forward<128, false, true",),
                "object_reduce": ("k_obj_reduce",), "loss": ("k_loss<",), "probe_reduce": ("k_probe_reduce",)}
     res = {"scale_fetch": round(sf, 4), "scale_write": round(sw, 4), "kernels": {}}
     for k, keys in kernels.items():
