@@ -737,6 +737,107 @@ __global__ __launch_bounds__(256) void k_obj_reduce_gen(Params p) {
   }
 }
 
+// Deterministic object-gradient scatter-add, one object row per block (used when the row fits the
+// LDS), one thread per patch column (blockDim.x = Nc rounded up to whole waves): the block lists, in
+// batch order, the patterns whose patch covers row y (ballot compaction); for each listed pattern
+// every thread sums the modes of its column of the covering patch row in order and adds the sum into
+// the W-wide LDS row.  One pattern's columns land on distinct object columns (W >= Nc), consecutive
+// patterns' may coincide, so a barrier separates the adds of consecutive patterns: every object pixel
+// sums its patterns in increasing b.  K patterns' loads (MC modes each) are issued together.
+template <int MC>
+__global__ __launch_bounds__(512) void k_obj_reduce_rows_gen(Params p) {
+  extern __shared__ cf acc_row[];  // p.W entries
+  __shared__ int lb[512], lrow[512], lc0[512];
+  __shared__ int wcount[8];
+  constexpr int K = 4;
+  const int t = threadIdx.x, TB = blockDim.x, nw = TB >> 6;
+  const int NR = p.NR, NC = p.NC, hr = NR / 2;
+  const size_t N2 = (size_t)NR * NC;
+  const int y = blockIdx.x, slot = blockIdx.y, s = p.s0 + slot;
+  const size_t slab = (size_t)p.wslots * N2;  // stride between modes of one pattern
+  const bool col_ok = t < NC;
+  const int ic = col_ok ? t : NC - 1;  // lanes past Nc read the row's last element and add nothing
+  const int xo = foff(ic, NC);         // the column's offset from the patch origin
+  for (int x = t; x < p.W; x += TB) acc_row[x] = mk(0.0f, 0.0f);
+  for (int b0 = 0; b0 < p.B; b0 += TB) {
+    const int b = b0 + t;
+    int u = NR, c0m = 0;
+    if (b < p.B) {
+      const int r0 = (int)rintf(p.pos[2 * b]), c0 = (int)rintf(p.pos[2 * b + 1]);
+      u = pmod(y - r0 + hr, p.H);
+      c0m = pmod(c0, p.W);
+    }
+    const bool hit = u < NR;
+    const unsigned long long bal = __ballot(hit);
+    const int lane = t & 63, wave = t >> 6;
+    const int pre = __popcll(bal & ((1ull << lane) - 1ull));
+    if (lane == 0) wcount[wave] = __popcll(bal);
+    __syncthreads();  // also orders the zeroing / the previous chunk's reads of the list
+    int off = 0, cnt = 0;
+    for (int w = 0; w < nw; ++w) {
+      off += w < wave ? wcount[w] : 0;
+      cnt += wcount[w];
+    }
+    if (hit) {
+      int row = u - hr;
+      row += row < 0 ? NR : 0;
+      lb[off + pre] = b;
+      lrow[off + pre] = row;
+      lc0[off + pre] = c0m;
+    }
+    __syncthreads();
+    for (int i0 = 0; i0 < cnt; i0 += K) {
+      cf val[K];
+      int xk[K];
+      size_t base[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int i = i0 + k < cnt ? i0 + k : i0;  // past the list's end: loaded, unused
+        base[k] = ((size_t)lb[i] * p.M * p.wslots + slot) * N2 + (size_t)lrow[i] * NC + ic;
+        int x = lc0[i] + xo;  // one wrap each way
+        x += x < 0 ? p.W : 0;
+        x -= x >= p.W ? p.W : 0;
+        xk[k] = x;
+        val[k] = mk(0.0f, 0.0f);
+      }
+      // the modes MC at a time: every load of a group is issued before its first add (mode indices
+      // past M - 1 re-read mode M - 1 and are not added), and the adds run in mode order
+      for (int m0 = 0; m0 < p.M; m0 += MC) {
+        cf v[K][MC];
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+#pragma unroll
+          for (int j = 0; j < MC; ++j) {
+            const int m = m0 + j < p.M ? m0 + j : p.M - 1;
+            v[k][j] = ld_stream(p.psi + base[k] + (size_t)m * slab);
+          }
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+#pragma unroll
+          for (int j = 0; j < MC; ++j) {
+            if (m0 + j < p.M) {
+              val[k].x += v[k][j].x;
+              val[k].y += v[k][j].y;
+            }
+          }
+      }
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        if (i0 + k < cnt) {  // uniform over the block
+          if (col_ok) {
+            acc_row[xk[k]].x += val[k].x;
+            acc_row[xk[k]].y += val[k].y;
+          }
+          __syncthreads();
+        }
+      }
+    }
+    __syncthreads();  // the list is rewritten by the next chunk
+  }
+  cf* out = p.grad_obj + ((size_t)s * p.H + y) * p.W;
+  for (int x = t; x < p.W; x += TB) out[x] = acc_row[x];
+}
+
 // gsum[m][k] = sum_b X[b][m][k] in a fixed order (4 lanes per frequency, every 4th pattern each,
 // combined in lane order).
 __global__ __launch_bounds__(256) void k_probe_reduce_gen(Params p) {
@@ -868,8 +969,20 @@ int launch_gen_adjoint(const Params& p, hipStream_t st) {
 }
 
 int launch_gen_obj_reduce(const Params& q, hipStream_t st) {
-  const dim3 g((q.W + 31) / 32, (q.H + 31) / 32, q.s1 - q.s0);
-  hipLaunchKernelGGL(gen::k_obj_reduce_gen, g, dim3(256), 0, st, q);
+  const size_t row_bytes = (size_t)q.W * sizeof(cf);
+  if (row_bytes <= 48 * 1024) {
+    const dim3 g(q.H, q.s1 - q.s0);
+    const dim3 blk((unsigned)((q.NC + 63) / 64 * 64));  // one thread per patch column, whole waves (<= 512)
+    if (q.M >= 4)
+      hipLaunchKernelGGL(gen::k_obj_reduce_rows_gen<4>, g, blk, row_bytes, st, q);
+    else if (q.M >= 2)
+      hipLaunchKernelGGL(gen::k_obj_reduce_rows_gen<2>, g, blk, row_bytes, st, q);
+    else
+      hipLaunchKernelGGL(gen::k_obj_reduce_rows_gen<1>, g, blk, row_bytes, st, q);
+  } else {
+    const dim3 g((q.W + 31) / 32, (q.H + 31) / 32, q.s1 - q.s0);
+    hipLaunchKernelGGL(gen::k_obj_reduce_gen, g, dim3(256), 0, st, q);
+  }
   return check_last_launch("object reduce (generic size)");
 }
 

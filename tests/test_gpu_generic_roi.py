@@ -66,7 +66,7 @@ def _oracle(case):
     "shape,s,m,b,descan",
     [((96, 96), 3, 2, 4, True), ((48, 80), 2, 1, 5, False), ((100, 120), 3, 3, 3, True), ((16, 16), 4, 2, 6, False),
      ((160, 192), 2, 2, 2, False), ((30, 42), 3, 2, 5, True), ((45, 75), 2, 1, 3, False), ((512, 20), 2, 1, 2, True),
-     ((8, 8), 2, 1, 4, False)],
+     ((8, 8), 2, 1, 4, False), ((24, 300), 2, 2, 3, True)],
 )
 def test_generic_shape_matches_oracle(cuda, shape, s, m, b, descan):
     print(f"generic shape={shape} s={s} m={m} b={b} descan={descan}", flush=True)
@@ -139,6 +139,17 @@ def test_generic_shape_batch_is_the_sum_of_its_parts(cuda):
     small["targets"] = case["targets"][:2]
     I = po.forward(small["obj"], small["probe"], small["positions"], small["propagators"], None)["I"]
     assert rel(full["I"][:2], I) < FWD_TOL
+
+
+def test_generic_shape_wide_object(cuda):
+    """An object row wider than the row-owner reduce's LDS row (6200 px): the tile reduce takes over."""
+    print("generic wide object (16, 24) W=6200", flush=True)
+    case = _case(N=(16, 24), S=2, M=2, B=4, seed=8, H=40, W=6200)
+    got = _run(case, cuda)
+    ref = _oracle(case)
+    assert abs(got["loss"] - ref["loss"]) / abs(ref["loss"]) < FWD_TOL
+    for k in ("obj", "probe", "positions"):
+        assert rel(got[k], ref[k]) < GRAD_TOL, k
 
 
 def test_projection_update_refuses_generic_shape(cuda):
