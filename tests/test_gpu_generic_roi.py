@@ -66,7 +66,7 @@ def _oracle(case):
     "shape,s,m,b,descan",
     [((96, 96), 3, 2, 4, True), ((48, 80), 2, 1, 5, False), ((100, 120), 3, 3, 3, True), ((16, 16), 4, 2, 6, False),
      ((160, 192), 2, 2, 2, False), ((30, 42), 3, 2, 5, True), ((45, 75), 2, 1, 3, False), ((512, 20), 2, 1, 2, True),
-     ((8, 8), 2, 1, 4, False), ((24, 300), 2, 2, 3, True)],
+     ((8, 8), 2, 1, 4, False), ((24, 300), 2, 2, 3, True), ((36, 50), 1, 2, 4, True)],
 )
 def test_generic_shape_matches_oracle(cuda, shape, s, m, b, descan):
     print(f"generic shape={shape} s={s} m={m} b={b} descan={descan}", flush=True)

@@ -1,6 +1,7 @@
 """The allocation-free forward (qem_ptycho_forward_loss_static) against the workspace forward
 (qem_ptycho_forward_loss): same kernels, so loss and intensities are bitwise equal; two calls in a row
-on the shared static workspace, the second with a smaller input, and a 256 x 256 case."""
+on the shared static workspace, the second with a smaller input, at 128 x 128, at 256 x 256 then 64 x 64,
+and at the generic shapes 96 x 80 then 30 x 42."""
 
 import ctypes
 
@@ -24,9 +25,10 @@ def _inputs(n, s, m, b, seed, dev):
     case = attach_targets(case, I)
     t = {k: torch.from_numpy(np.ascontiguousarray(case[k])).to(dev)
          for k in ("obj", "probe", "positions", "propagators", "targets", "mask")}
-    d = nat.QemDesc(num_slices=s, num_modes=m, batch=b, roi=n, obj_rows=case["obj"].shape[1],
+    nr, nc = (n, n) if isinstance(n, int) else n
+    d = nat.QemDesc(num_slices=s, num_modes=m, batch=b, roi=nr, obj_rows=case["obj"].shape[1],
                     obj_cols=case["obj"].shape[2], num_gpts=case["num_gpts"], loss_type=0,
-                    mean_intensity=case["mean_intensity"], flags=0, segment_slices=0)
+                    mean_intensity=case["mean_intensity"], flags=0, segment_slices=0, roi_cols=nc)
     return d, t
 
 
@@ -55,11 +57,12 @@ def _forward_static(d, t, dev):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("first,second", [((128, 8, 4, 40), (128, 3, 2, 9)), ((256, 3, 2, 12), (64, 2, 1, 5))])
+@pytest.mark.parametrize("first,second", [((128, 8, 4, 40), (128, 3, 2, 9)), ((256, 3, 2, 12), (64, 2, 1, 5)),
+                                          (((96, 80), 3, 2, 20), ((30, 42), 2, 1, 6))])
 def test_static_forward_matches_workspace_forward(cuda, first, second):
     for k, (n, s, m, b) in enumerate((first, second)):
-        print(f"static forward call {k}: n={n} s={s} m={m} b={b}", flush=True)
-        d, t = _inputs(n, s, m, b, 40 + k + n, cuda)
+        print(f"static forward call {k}: shape={n} s={s} m={m} b={b}", flush=True)
+        d, t = _inputs(n, s, m, b, 40 + k + (n if isinstance(n, int) else sum(n)), cuda)
         ref_loss, ref_I = _forward_workspace(d, t, cuda)
         loss, I = _forward_static(d, t, cuda)
         torch.cuda.synchronize()
