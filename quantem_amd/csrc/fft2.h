@@ -321,6 +321,22 @@ struct Fft2 {
   __device__ __forceinline__ int wv_off(int e) const {
     return rp * N + 2 * c0 + (e / RC1) * NRP * N + ((e % RC1) / 2) * 2 * RC2 + (e % 2);
   }
+  // Pointer to register e of a stored-wave slot (wv_off) at a workgroup-uniform base: the base, a
+  // 32-bit per-thread row offset (one register per row group e / RC1) and a constant below 1 KiB, so
+  // that an access is a scalar-base load / store with an immediate offset and needs no 64-bit address
+  // registers.
+  template <typename T>
+  __device__ __forceinline__ T* wv_ptr(T* base, int e) const {
+    const unsigned row = (unsigned)(rp * N + 2 * c0 + (e / RC1) * NRP * N) * 8u;
+    const unsigned col = (unsigned)(((e % RC1) / 2) * 2 * RC2 + (e % 2)) * 8u;
+    return (T*)((char*)base + (size_t)row + (size_t)col);
+  }
+  // Pointer to register e of an N x N row-major array (P3) at a workgroup-uniform base, formed like
+  // wv_ptr: the base plus a 32-bit per-thread offset.
+  template <typename T>
+  __device__ __forceinline__ T* p3_ptr(T* base, int e) const {
+    return (T*)((char*)base + (size_t)((unsigned)p3base[e / RR3] * 8u) + (size_t)(RR1 * RR2 * N * (e % RR3) * 8));
+  }
   // Offset kr*N + kc of register e in an N x N row-major array (P3).
   __device__ __forceinline__ int p3_off(int e) const { return p3base[e / RR3] + RR1 * RR2 * N * (e % RR3); }
 

@@ -147,6 +147,11 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
   // propagation (after its last exchange, before its final register pass), which hides their L2
   // latency; at smaller N other resident waves hide it and the loads stay at the slice start.
   constexpr int OK = N >= 128 ? PTS : 0;  // object values loaded ahead
+  // At N = 128 the propagator loads of the first QA registers (the first pass-3 register group) are
+  // issued before the slice's wave stores: loads and stores share one vmcnt counter, so loads issued
+  // after the stores (inside the transform) also waited for the stores.
+  constexpr int QA = N >= 128 ? F::RR3 : 0;
+  cf qa[QA > 0 ? QA : 1];
   cf o[PTS];
   oo.opaque();
 #pragma unroll
@@ -167,6 +172,14 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
 #pragma unroll
       for (int e = OK; e < PTS; ++e) o[e] = obj_at(O, oo.at(e));
     }
+    if constexpr (QA > 0) {
+      // the propagator values of the first QA registers are loaded before the wave stores; the last
+      // slice (no propagation) reads the object's first N x N values instead, unused
+      const cf* prq = s + 1 < p.s1 ? p.prop + (size_t)s * N * N : p.obj;
+#pragma unroll
+      for (int e = 0; e < QA; ++e) qa[e] = *f.p3_ptr(prq, e);
+      __builtin_amdgcn_sched_barrier(0);
+    }
     if (ALL || dst != nullptr) {
 #pragma unroll
       for (int e = 0; e < PTS; e += 2) st_stream2(dst + f.wv_off(e), v[e], v[e + 1]);
@@ -183,15 +196,17 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
       f.fwd_groups(
           v, lds,
           [&](auto h) {
+            if constexpr (h.value * F::RR3 >= QA) {
 #pragma unroll
-            for (int i = 0; i < F::RR3; ++i) q[i] = pr[f.p3_off(h.value * F::RR3 + i)];
+              for (int i = 0; i < F::RR3; ++i) q[i] = pr[f.p3_off(h.value * F::RR3 + i)];
+            }
           },
           [&](auto h) {
 #pragma unroll
             for (int i = 0; i < F::RR3; ++i) {
               const int e = h.value * F::RR3 + i;
               if constexpr (PG) st_stream(xs + f.p3_off(e), v[e]);
-              v[e] = cscale(cmul(v[e], q[i]), inv_n2);
+              v[e] = cscale(cmul(v[e], e < QA ? qa[e < QA ? e : 0] : q[i]), inv_n2);
             }
           });
       const cf* On = O + HW;
@@ -351,7 +366,7 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
       const float fr = fftfreq1<N>(row), fc = fftfreq1<N>(col);
       const cf ramp = cmul(expm2pi(fr * ddr), expm2pi(fc * ddc));
       if (want_desc) {
-        const cf xo = cmul(cmul(obj_at(O, oo.at(e)), ps[f.wv_off(e)]), ramp);
+        const cf xo = cmul(cmul(obj_at(O, oo.at(e)), *f.wv_ptr(ps, e)), ramp);
         const float im = cconjmul(v[e], xo).y;
         ar += (double)(fr * im);
         ac += (double)(fc * im);
@@ -380,17 +395,29 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
   constexpr bool AHEAD = PF == PTS;
   cf pre[PF > 0 ? PF : 1];
   cf ob[AHEAD ? PTS : 1];
+  // With every wave and object value prefetched (AHEAD), the slice step also issues the propagator
+  // loads of the slice's propagation, before its patch stores: loads and stores share one vmcnt
+  // counter, so propagator loads issued after the stores (inside the transform) waited for them.
+  cf qa[AHEAD ? F::RR3 : 1];
   auto slice_step = [&](int s, bool have_pre) {
     const cf* O = p.obj + (size_t)s * HW;
     cf* ps = p.psi + ((size_t)item * p.wslots + (s - p.s0)) * N * N;
     oo.opaque();
     if constexpr (AHEAD) {
+      // the patch values conj(psi_s) g_x go into pre, g_psi = conj(O_s) g_x into v
 #pragma unroll
-      for (int i = 0; i < PTS; i += 2) {
-        st_stream2(ps + f.wv_off(i), cconjmul(pre[i], v[i]), cconjmul(pre[i + 1], v[i + 1]));
+      for (int i = 0; i < PTS; ++i) {
+        pre[i] = cconjmul(pre[i], v[i]);
         v[i] = cconjmul(ob[i], v[i]);
-        v[i + 1] = cconjmul(ob[i + 1], v[i + 1]);
       }
+      __builtin_amdgcn_sched_barrier(0);  // the object values are dead before the propagator loads
+      // slice 0 has no propagation: its loads read the object's first N x N values, unused
+      const cf* prq = s > 0 ? p.prop + (size_t)(s - 1) * N * N : p.obj;
+#pragma unroll
+      for (int i = 0; i < F::RR3; ++i) qa[i] = *f.p3_ptr(prq, i);
+      __builtin_amdgcn_sched_barrier(0);  // the first register group's loads are issued before the stores
+#pragma unroll
+      for (int i = 0; i < PTS; i += 2) st_stream2(f.wv_ptr(ps, i), pre[i], pre[i + 1]);
     } else {
 #pragma unroll
       for (int e0 = 0; e0 < PTS; e0 += CH) {
@@ -400,14 +427,14 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
           if (e0 + i < PF && have_pre) {
             t[i] = pre[e0 + i];
           } else if (i % 2 == 0) {
-            ld_stream2(ps + f.wv_off(e0 + i), t[i], t[i + 1]);
+            ld_stream2(f.wv_ptr(ps, e0 + i), t[i], t[i + 1]);
           }
           o[i] = obj_at(O, oo.at(e0 + i));
         }
 #pragma unroll
         for (int i = 0; i < CH; ++i) {
           if (i % 2 == 0)
-            st_stream2(ps + f.wv_off(e0 + i), cconjmul(t[i], v[e0 + i]), cconjmul(t[i + 1], v[e0 + i + 1]));
+            st_stream2(f.wv_ptr(ps, e0 + i), cconjmul(t[i], v[e0 + i]), cconjmul(t[i + 1], v[e0 + i + 1]));
           v[e0 + i] = cconjmul(o[i], v[e0 + i]);
         }
       }
@@ -417,7 +444,7 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
     // the top slice's psi and object values, loaded as every later slice's are prefetched
     const cf* pt = p.psi + ((size_t)item * p.wslots + (p.s1 - 1 - p.s0)) * N * N;
 #pragma unroll
-    for (int i = 0; i < PF; i += 2) ld_stream2(pt + f.wv_off(i), pre[i], pre[i + 1]);
+    for (int i = 0; i < PF; i += 2) ld_stream2(f.wv_ptr(pt, i), pre[i], pre[i + 1]);
     oo.opaque();
 #pragma unroll
     for (int i = 0; i < PTS; ++i) ob[i] = obj_at(p.obj + (size_t)(p.s1 - 1) * HW, oo.at(i));
@@ -434,11 +461,13 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
       f.fwd_groups(
           v, lds,
           [&](auto h) {
+            if constexpr (!AHEAD || h.value > 0) {
 #pragma unroll
-            for (int i = 0; i < F::RR3; ++i) q[i] = pr[f.p3_off(h.value * F::RR3 + i)];
+              for (int i = 0; i < F::RR3; ++i) q[i] = *f.p3_ptr(pr, h.value * F::RR3 + i);
+            }
             if constexpr (PG) {
 #pragma unroll
-              for (int i = 0; i < F::RR3; ++i) xq[i] = ld_stream(xs + f.p3_off(h.value * F::RR3 + i));
+              for (int i = 0; i < F::RR3; ++i) xq[i] = ld_stream(f.p3_ptr(xs, h.value * F::RR3 + i));
             }
           },
           [&](auto h) {
@@ -446,8 +475,8 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
             for (int i = 0; i < F::RR3; ++i) {
               const int e = h.value * F::RR3 + i;
               // dL/dP_{s-1} of this item: conj(fft2(x_{s-1})) fft2(g_psi_s) / N^2, over fft2(x_{s-1})
-              if constexpr (PG) st_stream(xs + f.p3_off(e), cscale(cconjmul(xq[i], v[e]), inv_n2));
-              v[e] = cscale(cmulc(v[e], q[i]), inv_n2);
+              if constexpr (PG) st_stream(f.p3_ptr(xs, e), cscale(cconjmul(xq[i], v[e]), inv_n2));
+              v[e] = cscale(cmulc(v[e], AHEAD && h.value == 0 ? qa[i] : q[i]), inv_n2);
             }
           });
       if constexpr (PF > 0) {
@@ -457,7 +486,7 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
         const int sl = s - 1 >= p.s0 ? s - 1 - p.s0 : 0;
         const cf* pn = p.psi + ((size_t)item * p.wslots + sl) * N * N;
 #pragma unroll
-        for (int i = 0; i < PF; i += 2) ld_stream2(pn + f.wv_off(i), pre[i], pre[i + 1]);
+        for (int i = 0; i < PF; i += 2) ld_stream2(f.wv_ptr(pn, i), pre[i], pre[i + 1]);
       }
       const cf* On = p.obj + (size_t)(s - 1) * HW;
       f.inv_hooked(v, lds, [&] {

@@ -17,6 +17,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "common.h"
 #include "ptycho_common.h"
 
@@ -27,14 +29,11 @@ constexpr int N = 256;
 constexpr int N2 = N * N;
 // Threads per workgroup: 512 (two waves per SIMD, up to 256 VGPRs each, 32 rows or columns per chunk)
 // measured 20 % faster in the forward and 6 % in the adjoint than 256 threads with the next chunk's
-// values carried in registers (EARLY_NEXT); 1024 threads (128 VGPRs) spill and ran slower.
+// values carried in registers; 1024 threads (128 VGPRs) spill and ran slower.
 constexpr int T = 512;
 constexpr int CH = T / 16;   // rows (or columns) per chunk: 16 lanes per row
 constexpr int VB = 272;      // LDS block per 16-lane vector: 16 x 17 (rows) or 16 x 16 + 16 (columns)
 static_assert(T % 256 == 0 && T <= 1024, "whole 16-column groups in the column passes");
-// Carrying the next chunk's values in registers (EARLY_NEXT) pays with one wave per SIMD; with two the
-// other wave hides the load latency and the registers are needed for the chunk itself.
-constexpr bool kCarry = T <= 256;
 
 struct Smem {
   cf ex[CH * VB];  // exchange of the 16 x 16 decomposition
@@ -45,12 +44,83 @@ struct Smem {
 
 // The multislice kernels run one workgroup per CU: with 256 items resident the row-transformed waves
 // (256 x 512 KiB) stay in the Infinity Cache across the two passes of every slice, which measured
-// faster than two or three items per CU streaming twice that through HBM.  An LDS footprint above
-// half a CU's 160 KiB enforces it (the exchange buffer of 32 vectors is 68 KiB; the rest is padding).
+// faster than two or three items per CU streaming twice that through HBM.  Their LDS holds, besides
+// the exchange buffer, one chunk of W (64 KiB) that the LDS DMA copies in while the previous chunk is
+// transformed; the footprint above half a CU's 160 KiB also keeps one workgroup per CU.
 struct SmemOne : Smem {
-  char occupancy_pad[sizeof(Smem) > 82 * 1024 ? 1 : 82 * 1024 - sizeof(Smem)];
+  cf dma[CH * N];  // the next chunk of W: rows [e][vector][j] or columns [e][j][column] (see dma_rows / dma_cols)
 };
-static_assert(sizeof(SmemOne) > 80 * 1024, "one workgroup per CU");
+static_assert(sizeof(SmemOne) > 80 * 1024 && sizeof(SmemOne) <= 160 * 1024, "one workgroup per CU");
+
+// LDS DMA (global_load_lds_dwordx4): every lane copies 16 bytes from its own global address to
+// consecutive 16-byte slots of a 1 KiB LDS block.  A chunk of W (CH vectors x 256 values, 64 KiB) is
+// 64 such blocks, 8 per wave.  The copy bypasses the registers, so the next chunk is fetched while the
+// current one is transformed.  The waits are explicit: a pass waits with vmcnt(16) for its own copy,
+// allowing only the previous chunk's 16 row / column stores (the last vector-memory instructions of
+// the wave) to stay in flight; loads and stores share the one in-order vmcnt counter.
+typedef __attribute__((address_space(3))) void* lds_void_ptr;
+
+// One copy instruction: 16 bytes per lane from base + lane_off to the LDS block at lds (its 64 lanes'
+// data consecutive).  Written as inline assembly so that the compiler's wait-count pass does not see
+// it: it treats an LDS DMA as pending until vmcnt(0) and waited for every outstanding store before
+// any read of the block; the passes wait for their copies explicitly (dma_wait).  The instruction
+// writes no register.
+__device__ __forceinline__ void lds_dma16(const void* base, unsigned lane_off, const cf* lds) {
+  const unsigned m0 = __builtin_amdgcn_readfirstlane((unsigned)(size_t)(lds_void_ptr)lds);  // LDS block
+  const char* addr = (const char*)base + lane_off;
+  asm volatile("s_nop 0\n\tglobal_load_lds_dwordx4 %0, off" : : "v"(addr), "{m0}"(m0) : "memory");
+}
+
+// Row chunk [r0, r0 + CH) of src into dma laid out [e][vector][j] (value j + 16 e of row r0 + vector):
+// a wave's read of its v[e] is then 512 contiguous bytes.  The wave index is made scalar, so that the
+// block offsets are scalar and each copy instruction is a scalar base plus one 32-bit lane offset.
+__device__ __forceinline__ void dma_rows(const cf* src, int r0, SmemOne& sm) {
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int l = threadIdx.x & 63;
+  unsigned lane = (unsigned)((l >> 3) * N + 2 * (l & 7)) * 8u;  // row (l / 8) of the block, values 2 (l % 8), +1
+  asm volatile("" : "+v"(lane));  // formed here, not kept live through the transforms
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int blk = w * 8 + i;  // e = blk / 4, vectors (blk % 4) * 8 .. + 7
+    const unsigned ub = (unsigned)((r0 + (blk & 3) * 8) * N + 16 * (blk >> 2)) * 8u;
+    lds_dma16(src, ub + lane, &sm.dma[blk * 128]);
+  }
+}
+__device__ __forceinline__ cf dma_row_value(const SmemOne& sm, int e) {
+  return sm.dma[(e * CH + (int)(threadIdx.x >> 4)) * 16 + (int)(threadIdx.x & 15)];
+}
+// Column chunk [c0, c0 + CH) of src into dma laid out [e][j][column ^ 16 (j & 1)] (value j + 16 e of
+// column c0 + column): the swizzle spreads a wave's four j rows of 128 bytes over both halves of the
+// banks.
+__device__ __forceinline__ void dma_cols(const cf* src, int c0, SmemOne& sm) {
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int l = threadIdx.x & 63;
+  // lane: j offset l / 16 within the block's four j, columns 2 (l % 16) (+1), swizzled for odd j
+  unsigned lane = (unsigned)((l >> 4) * N + ((2 * (l & 15)) ^ (16 * ((l >> 4) & 1)))) * 8u;
+  asm volatile("" : "+v"(lane));
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int blk = w * 8 + i;  // e = blk / 4, j = (blk % 4) * 4 .. + 3 (j parity = lane's l / 16 parity)
+    const unsigned ub = (unsigned)(((blk & 3) * 4 + 16 * (blk >> 2)) * N + c0) * 8u;
+    lds_dma16(src, ub + lane, &sm.dma[blk * 128]);
+  }
+}
+__device__ __forceinline__ cf dma_col_value(const SmemOne& sm, int e) {
+  const int j = (threadIdx.x >> 4) & 15;
+  const int c = (threadIdx.x & 15) + 16 * (threadIdx.x >> 8);
+  return sm.dma[((e * 16 + j) * CH) + (c ^ (16 * (j & 1)))];
+}
+// Waits for this wave's part of the chunk copy, then for every wave's (barrier).  `stores` = the
+// number of vector-memory instructions issued after the copy (the previous chunk's stores), which may
+// stay in flight: vmcnt(16) or vmcnt(0).
+__device__ __forceinline__ void dma_wait(bool stores) {
+  if (stores)
+    __builtin_amdgcn_s_waitcnt(0x4F70);  // vmcnt(16)
+  else
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 
 // Lane roles.  Row passes: vector (row) t / 16, j = t % 16.  Column passes: column t % 16 + 16 (t / 256)
 // (one 16-column group per 256 threads), j = (t / 16) % 16.  In both, the lane holds elements
@@ -97,55 +167,26 @@ struct NoPre {
   __device__ __forceinline__ void operator()(int) const {}
 };
 
-// Row pass over the 256 rows of src (CH rows per chunk): v <- row (columns j + 16 e); inverse row DFT
-// if PRE_INV; op(r, v) on the natural-order row; forward row DFT if POST_FWD; row -> dst (if not null).
-// EARLY_NEXT: each chunk issues the next chunk's loads before its own stores (a load issued after a
-// store waits for it: one vmcnt counter) and carries them in registers.  Measured faster for both
-// passes of the adjoint and for the forward's row pass; with it in the forward's column pass as well
-// the forward's registers spill into AGPRs and it runs slower.  pre(r) (EARLY_NEXT only) runs for the
-// first chunk's rows before the loop and for each next chunk's rows right after that chunk's loads,
-// so that op can use values it loaded one chunk ahead (their latency then hides behind the next
-// chunk's inverse row DFT).
-template <bool PRE_INV, bool POST_FWD, bool EARLY_NEXT = false, typename Op, typename Pre = NoPre>
+// Row pass over the 256 rows of src (CH rows per chunk): v <- row (columns j + 16 e); pre(r) (the op's
+// own loads, issued before the inverse row DFT so that it hides their latency); inverse row DFT if
+// PRE_INV; op(r, v) on the natural-order row; forward row DFT if POST_FWD; row -> dst (if not null).
+// This form serves the single-pass kernels (probe spectra, probe gradient, centring); the multislice
+// kernels use the overloads below, which stage every chunk of W through LDS.
+template <bool PRE_INV, bool POST_FWD, typename Op, typename Pre = NoPre>
 __device__ __forceinline__ void row_pass(const cf* src, cf* dst, Smem& sm, Op&& op, Pre&& pre = Pre()) {
   const int j = row_j();
-  if constexpr (EARLY_NEXT && kCarry) {
+  for (int r0 = 0; r0 < N; r0 += CH) {
+    const int r = r0 + row_vec();
     cf v[16];
 #pragma unroll
-    for (int e = 0; e < 16; ++e) v[e] = src[row_vec() * N + j + 16 * e];
-    pre(row_vec());
-    for (int r0 = 0; r0 < N; r0 += CH) {
-      const int r = r0 + row_vec();
-      if constexpr (PRE_INV) fft256<true, false>(v, sm);
-      op(r, v);
-      if constexpr (POST_FWD) fft256<false, false>(v, sm);
-      cf nxt[16];
-      if (r0 + CH < N) {  // the chunks touch disjoint rows
+    for (int e = 0; e < 16; ++e) v[e] = src[r * N + j + 16 * e];
+    pre(r);
+    if constexpr (PRE_INV) fft256<true, false>(v, sm);
+    op(r, v);
+    if constexpr (POST_FWD) fft256<false, false>(v, sm);
+    if (dst != nullptr) {
 #pragma unroll
-        for (int e = 0; e < 16; ++e) nxt[e] = src[(r + CH) * N + j + 16 * e];
-        pre(r + CH);
-      }
-      if (dst != nullptr) {
-#pragma unroll
-        for (int e = 0; e < 16; ++e) dst[r * N + j + 16 * e] = v[e];
-      }
-#pragma unroll
-      for (int e = 0; e < 16; ++e) v[e] = nxt[e];
-    }
-  } else {
-    for (int r0 = 0; r0 < N; r0 += CH) {
-      const int r = r0 + row_vec();
-      cf v[16];
-#pragma unroll
-      for (int e = 0; e < 16; ++e) v[e] = src[r * N + j + 16 * e];
-      pre(r);  // the chunk's own values for op, loaded before its inverse row DFT
-      if constexpr (PRE_INV) fft256<true, false>(v, sm);
-      op(r, v);
-      if constexpr (POST_FWD) fft256<false, false>(v, sm);
-      if (dst != nullptr) {
-#pragma unroll
-        for (int e = 0; e < 16; ++e) dst[r * N + j + 16 * e] = v[e];
-      }
+      for (int e = 0; e < 16; ++e) dst[r * N + j + 16 * e] = v[e];
     }
   }
   __syncthreads();  // dst complete and visible to the whole workgroup
@@ -153,36 +194,79 @@ __device__ __forceinline__ void row_pass(const cf* src, cf* dst, Smem& sm, Op&& 
 
 // Column pass over the 256 columns of src (CH columns per chunk): v <- column c (rows j + 16 e);
 // op(c, v) (which runs the column DFTs it needs); column -> dst.
-template <bool EARLY_NEXT = false, typename Op>
+template <typename Op>
 __device__ __forceinline__ void col_pass(const cf* src, cf* dst, Smem& sm, Op&& op) {
   const int j = col_j();
-  if constexpr (EARLY_NEXT && kCarry) {
+  for (int c0 = 0; c0 < N; c0 += CH) {
+    const int c = c0 + col_vec();
     cf v[16];
 #pragma unroll
-    for (int e = 0; e < 16; ++e) v[e] = src[(j + 16 * e) * N + col_vec()];
-    for (int c0 = 0; c0 < N; c0 += CH) {
-      const int c = c0 + col_vec();
-      op(c, v);
-      cf nxt[16];
-      if (c0 + CH < N) {  // the chunks touch disjoint columns
+    for (int e = 0; e < 16; ++e) v[e] = src[(j + 16 * e) * N + c];
+    op(c, v);
 #pragma unroll
-        for (int e = 0; e < 16; ++e) nxt[e] = src[(j + 16 * e) * N + c + CH];
-      }
+    for (int e = 0; e < 16; ++e) dst[(j + 16 * e) * N + c] = v[e];
+  }
+  __syncthreads();
+}
+
+// row_pass / col_pass of the multislice kernels (one workgroup per CU, SmemOne): the chunk's W values
+// come from the LDS copy of it started during the previous chunk.  Per chunk: wait for the copy, read
+// it, pre (the op's own global loads), then the transforms and op; the next chunk's copy is started
+// once op has used its loads (a load still in flight when the copy is issued would be waited for
+// together with the copy) and runs during the rest of the chunk's transforms and its stores.
+template <bool PRE_INV, bool POST_FWD, typename Op, typename Pre = NoPre>
+__device__ __forceinline__ void row_pass(const cf* src, cf* dst, SmemOne& sm, Op&& op, Pre&& pre = Pre()) {
+  const int j = row_j();
+  dma_rows(src, 0, sm);
+  for (int r0 = 0; r0 < N; r0 += CH) {
+    const int r = r0 + row_vec();
+    dma_wait(r0 > 0 && dst != nullptr);
+    cf v[16];
 #pragma unroll
-      for (int e = 0; e < 16; ++e) dst[(j + 16 * e) * N + c] = v[e];
+    for (int e = 0; e < 16; ++e) v[e] = dma_row_value(sm, e);
+    lds_barrier();  // every wave has read the chunk before the next copy overwrites it
+    pre(r);
+    if constexpr (PRE_INV) fft256<true, false>(v, sm);
+    op(r, v);
+    __builtin_amdgcn_sched_barrier(0);  // op's use of its loads stays before the copy
+    if (r0 + CH < N) dma_rows(src, r0 + CH, sm);
+    __builtin_amdgcn_sched_barrier(0);  // and the copy is issued before the transform
+    if constexpr (POST_FWD) fft256<false, false>(v, sm);
+    if (dst != nullptr) {
 #pragma unroll
-      for (int e = 0; e < 16; ++e) v[e] = nxt[e];
+      for (int e = 0; e < 16; ++e) dst[r * N + j + 16 * e] = v[e];
     }
-  } else {
-    for (int c0 = 0; c0 < N; c0 += CH) {
-      const int c = c0 + col_vec();
-      cf v[16];
+  }
+  __syncthreads();  // dst complete and visible to the whole workgroup
+}
+
+// op(c, v, next) calls next() where the next chunk's copy may start (after its last use of a global
+// load); an op without that parameter gets the copy started after it returns.
+template <typename Op, typename Pre = NoPre>
+__device__ __forceinline__ void col_pass(const cf* src, cf* dst, SmemOne& sm, Op&& op, Pre&& pre = Pre()) {
+  const int j = col_j();
+  dma_cols(src, 0, sm);
+  for (int c0 = 0; c0 < N; c0 += CH) {
+    const int c = c0 + col_vec();
+    dma_wait(c0 > 0);
+    cf v[16];
 #pragma unroll
-      for (int e = 0; e < 16; ++e) v[e] = src[(j + 16 * e) * N + c];
+    for (int e = 0; e < 16; ++e) v[e] = dma_col_value(sm, e);
+    lds_barrier();  // every wave has read the chunk before the next copy overwrites it
+    pre(c);
+    auto next = [&] {
+      __builtin_amdgcn_sched_barrier(0);  // op's use of its loads stays before the copy
+      if (c0 + CH < N) dma_cols(src, c0 + CH, sm);
+      __builtin_amdgcn_sched_barrier(0);  // and the copy is issued before what follows
+    };
+    if constexpr (std::is_invocable_v<Op, int, cf(&)[16], decltype(next)&>) {
+      op(c, v, next);
+    } else {
       op(c, v);
-#pragma unroll
-      for (int e = 0; e < 16; ++e) dst[(j + 16 * e) * N + c] = v[e];
+      next();
     }
+#pragma unroll
+    for (int e = 0; e < 16; ++e) dst[(j + 16 * e) * N + c] = v[e];
   }
   __syncthreads();
 }
@@ -262,13 +346,20 @@ __global__ __launch_bounds__(T) void k_forward256(Params p) {
   if (far) {
     // psi_0 = ifft2(fft2(P_m) * ramp_b): inverse column DFT here, inverse row DFT in the first row pass
     const cf* rr = p.ramps + (size_t)b * 2 * N;  // row factors, then column factors
-    col_pass(p.pk + (size_t)m * N2, Wb, sm, [&](int kc, cf(&v)[16]) {
-      const cf rc = rr[N + kc];
-      const int jj = col_j();
+    cf rw[16], rcol;
+    col_pass(
+        p.pk + (size_t)m * N2, Wb, sm,
+        [&](int, cf(&v)[16], auto&& next) {
 #pragma unroll
-      for (int e = 0; e < 16; ++e) v[e] = cscale(cmul(v[e], cmul(rr[jj + 16 * e], rc)), inv_n2);
-      fft256<true, true>(v, sm);
-    });
+          for (int e = 0; e < 16; ++e) v[e] = cscale(cmul(v[e], cmul(rw[e], rcol)), inv_n2);
+          next();
+          fft256<true, true>(v, sm);
+        },
+        [&](int kc) {
+          rcol = rr[N + kc];
+#pragma unroll
+          for (int e = 0; e < 16; ++e) rw[e] = rr[col_j() + 16 * e];
+        });
   }
 
   for (int s = p.s0; s < p.s1; ++s) {
@@ -276,8 +367,9 @@ __global__ __launch_bounds__(T) void k_forward256(Params p) {
     cf* keep = slot(s);
     const bool transmit = far || s + 1 < p.s1;  // a recomputed segment needs only its incident waves
     const bool exit_ramp = DESCAN && far && s == p.S - 1;
-    // the transmit pass (op_t) gets its object values one chunk ahead (pre_obj); the other row passes
-    // (a recomputed segment's first and last slice) use op, which loads them in the chunk itself
+    // the transmit pass (op_t) gets its object values from pre_obj, issued before the chunk's inverse
+    // row DFT; the other row passes (a recomputed segment's first and last slice) use op, which loads
+    // them after it
     cf ov[16];
     auto pre_obj = [&](int r) {
       const unsigned orow = obj_row_off(r0, r, p.H, p.W);
@@ -329,27 +421,36 @@ __global__ __launch_bounds__(T) void k_forward256(Params p) {
       else
         row_pass<false, false>(ck, nullptr, sm, op);
     } else if (transmit) {
-      row_pass<true, true, true>(Wb, Wb, sm, op_t, pre_obj);
+      row_pass<true, true>(Wb, Wb, sm, op_t, pre_obj);
     } else {
       row_pass<true, false>(Wb, nullptr, sm, op);
     }
     if (s + 1 < p.s1) {
       const cf* pr = p.prop + (size_t)s * N2;
       cf* xs = PG ? p.pspec + ((size_t)item * (p.S - 1) + s) * N2 : nullptr;
-      col_pass(Wb, Wb, sm, [&](int kc, cf(&v)[16]) {
-        fft256<false, true>(v, sm);
-        const int jj = col_j();
-        if constexpr (PG) {  // fft2 of the transmitted wave, for the propagator gradient
+      cf pq[16];
+      col_pass(
+          Wb, Wb, sm,
+          [&](int kc, cf(&v)[16], auto&& next) {
+            fft256<false, true>(v, sm);
+            const int jj = col_j();
+            if constexpr (PG) {  // fft2 of the transmitted wave, for the propagator gradient
 #pragma unroll
-          for (int e = 0; e < 16; ++e) st_stream(xs + (jj + 16 * e) * N + kc, v[e]);
-        }
+              for (int e = 0; e < 16; ++e) st_stream(xs + (jj + 16 * e) * N + kc, v[e]);
+            }
 #pragma unroll
-        for (int e = 0; e < 16; ++e) v[e] = cscale(cmul(v[e], pr[(jj + 16 * e) * N + kc]), inv_n2);
-        fft256<true, true>(v, sm);
-      });
+            for (int e = 0; e < 16; ++e) v[e] = cscale(cmul(v[e], pq[e]), inv_n2);
+            next();
+            fft256<true, true>(v, sm);
+          },
+          [&](int kc) {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) pq[e] = pr[(col_j() + 16 * e) * N + kc];
+          });
     } else if (far) {
       const float inv_n = 1.0f / (float)N;
-      col_pass(Wb, p.X + (size_t)item * N2, sm, [&](int, cf(&v)[16]) {
+      col_pass(Wb, p.X + (size_t)item * N2, sm, [&](int, cf(&v)[16], auto&& next) {
+        next();
         fft256<false, true>(v, sm);
 #pragma unroll
         for (int e = 0; e < 16; ++e) v[e] = cscale(v[e], inv_n);
@@ -383,7 +484,8 @@ __global__ __launch_bounds__(T) void k_adjoint256(Params p) {
   if (top && analytic) {
     // the projection kernel left the Fourier-space residual X' - X over X: g = ifft2_ortho(residual)
     const float inv_n = 1.0f / (float)N;
-    col_pass(Xs, Wb, sm, [&](int, cf(&v)[16]) {
+    col_pass(Xs, Wb, sm, [&](int, cf(&v)[16], auto&& next) {
+      next();
 #pragma unroll
       for (int e = 0; e < 16; ++e) v[e] = cscale(v[e], inv_n);
       fft256<true, true>(v, sm);
@@ -392,12 +494,19 @@ __global__ __launch_bounds__(T) void k_adjoint256(Params p) {
     // g_x = (1/N) F^H (2 dL/dI X): inverse column DFT here, inverse row DFT in the first row pass
     const float* gI = p.gI + (size_t)b * N2;
     const float sc = 2.0f * p.dloss[0] / (float)N;
-    col_pass(Xs, Wb, sm, [&](int kc, cf(&v)[16]) {
-      const int jj = col_j();
+    float gv[16];
+    col_pass(
+        Xs, Wb, sm,
+        [&](int, cf(&v)[16], auto&& next) {
 #pragma unroll
-      for (int e = 0; e < 16; ++e) v[e] = cscale(v[e], sc * gI[(jj + 16 * e) * N + kc]);
-      fft256<true, true>(v, sm);
-    });
+          for (int e = 0; e < 16; ++e) v[e] = cscale(v[e], sc * gv[e]);
+          next();
+          fft256<true, true>(v, sm);
+        },
+        [&](int kc) {
+#pragma unroll
+          for (int e = 0; e < 16; ++e) gv[e] = gI[(col_j() + 16 * e) * N + kc];
+        });
   }
 
   double dar = 0.0, dac = 0.0;
@@ -433,26 +542,25 @@ __global__ __launch_bounds__(T) void k_adjoint256(Params p) {
   for (int s = p.s1 - 1; s >= p.s0; --s) {
     const cf* O = p.obj + (size_t)s * HW;
     cf* A = p.psi + ((size_t)item * p.wslots + (s - p.s0)) * N2;
-    // g_x of slice s in natural order: object-gradient patch conj(psi_s) g over psi_s, g_psi = conj(O_s) g
-    // the stored-wave values of a row chunk are loaded one chunk ahead (pre); the object values
-    // stay in the chunk itself: loading both ahead pushed the kernel into AGPR copies (slower)
-    cf pv[16];
-    row_pass<true, true, true>(
+    // g_x of slice s in natural order: object-gradient patch conj(psi_s) g over psi_s, g_psi = conj(O_s) g;
+    // a row chunk's stored-wave and object values are loaded by pre, before its inverse row DFT
+    // (loading them a whole chunk ahead needs 64 more registers and measured slower)
+    cf pv[16], ov[16];
+    row_pass<true, true>(
         (!top && s == p.s1 - 1) ? (const cf*)Xs : (const cf*)Wb, Wb, sm,
         [&](int r, cf(&v)[16]) {
-          const unsigned orow = obj_row_off(r0, r, p.H, p.W);
-          cf o[16];
-#pragma unroll
-          for (int e = 0; e < 16; ++e) o[e] = obj_at(O, orow + obj_col_off(q, e, p.W));
 #pragma unroll
           for (int e = 0; e < 16; ++e) {
             st_stream(A + r * N + j + 16 * e, cconjmul(pv[e], v[e]));
-            v[e] = cconjmul(o[e], v[e]);
+            v[e] = cconjmul(ov[e], v[e]);
           }
         },
         [&](int r) {
 #pragma unroll
           for (int e = 0; e < 16; ++e) pv[e] = ld_stream(A + r * N + j + 16 * e);
+          const unsigned orow = obj_row_off(r0, r, p.H, p.W);
+#pragma unroll
+          for (int e = 0; e < 16; ++e) ov[e] = obj_at(O, orow + obj_col_off(q, e, p.W));
         });
     if (s > 0) {
       // g_x <- ifft2(fft2(g_psi) conj(prop_{s-1})); at a segment bottom it goes to X for the segment below
@@ -460,16 +568,12 @@ __global__ __launch_bounds__(T) void k_adjoint256(Params p) {
       // (the propagator column is loaded before the column DFT, which hides its latency: measured
       // faster here, slower in the forward's column pass)
       cf* xs = PG ? p.pspec + ((size_t)item * (p.S - 1) + (s - 1)) * N2 : nullptr;
-      col_pass<true>(Wb, s == p.s0 ? Xs : Wb, sm, [&](int kc, cf(&v)[16]) {
+      cf q[16];
+      cf xq[PG ? 16 : 1];
+      col_pass(
+          Wb, s == p.s0 ? Xs : Wb, sm,
+          [&](int kc, cf(&v)[16], auto&& next) {
         const int jj = col_j();
-        cf q[16];
-        cf xq[PG ? 16 : 1];
-#pragma unroll
-        for (int e = 0; e < 16; ++e) q[e] = pr[(jj + 16 * e) * N + kc];
-        if constexpr (PG) {
-#pragma unroll
-          for (int e = 0; e < 16; ++e) xq[e] = ld_stream(xs + (jj + 16 * e) * N + kc);
-        }
         fft256<false, true>(v, sm);
         if constexpr (PG) {  // this item's dL/dP_{s-1}: conj(fft2(x_{s-1})) fft2(g_psi_s) / N^2
 #pragma unroll
@@ -477,8 +581,17 @@ __global__ __launch_bounds__(T) void k_adjoint256(Params p) {
         }
 #pragma unroll
         for (int e = 0; e < 16; ++e) v[e] = cscale(cmulc(v[e], q[e]), inv_n2);
+        next();
         fft256<true, true>(v, sm);
-      });
+          },
+          [&](int kc) {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) q[e] = pr[(col_j() + 16 * e) * N + kc];
+            if constexpr (PG) {
+#pragma unroll
+              for (int e = 0; e < 16; ++e) xq[e] = ld_stream(xs + (col_j() + 16 * e) * N + kc);
+            }
+          });
     }
   }
   if constexpr (DESCAN) {
@@ -494,7 +607,10 @@ __global__ __launch_bounds__(T) void k_adjoint256(Params p) {
   if (p.s0 > 0) return;
   if (analytic) {
     // the reference sums g at the probe plane without the shift's adjoint: G = fft2(g_psi_0) over X
-    col_pass(Wb, Xs, sm, [&](int, cf(&v)[16]) { fft256<false, true>(v, sm); });
+    col_pass(Wb, Xs, sm, [&](int, cf(&v)[16], auto&& next) {
+      next();
+      fft256<false, true>(v, sm);
+    });
     return;
   }
   // G = fft2(g_psi_0); probe-gradient spectrum conj(ramp_b) G over X, position partials
@@ -502,23 +618,34 @@ __global__ __launch_bounds__(T) void k_adjoint256(Params p) {
   const cf* pk = p.pk + (size_t)m * N2;
   const cf* rr = p.ramps + (size_t)b * 2 * N;  // row factors, then column factors
   double ar = 0.0, ac = 0.0;
-  col_pass(Wb, Xs, sm, [&](int kc, cf(&v)[16]) {
-    fft256<false, true>(v, sm);
-    const int jj = col_j();
-    const float fc = fftfreq1<N>(kc);
-    const cf rc = rr[N + kc];
+  cf pkv[16], rrv[16], rcv;
+  col_pass(
+      Wb, Xs, sm,
+      [&](int kc, cf(&v)[16]) {
+        fft256<false, true>(v, sm);
+        const int jj = col_j();
+        const float fc = fftfreq1<N>(kc);
 #pragma unroll
-    for (int e = 0; e < 16; ++e) {
-      const int kr = jj + 16 * e;
-      const float fr = fftfreq1<N>(kr);
-      const cf R = cmul(rr[kr], rc);
-      const cf G = v[e];
-      const float im = cconjmul(G, cmul(pk[kr * N + kc], R)).y;
-      ar += (double)(fr * im);
-      ac += (double)(fc * im);
-      v[e] = cmulc(G, R);
-    }
-  });
+        for (int e = 0; e < 16; ++e) {
+          const int kr = jj + 16 * e;
+          const float fr = fftfreq1<N>(kr);
+          const cf R = cmul(rrv[e], rcv);
+          const cf G = v[e];
+          const float im = cconjmul(G, cmul(pkv[e], R)).y;
+          ar += (double)(fr * im);
+          ac += (double)(fc * im);
+          v[e] = cmulc(G, R);
+        }
+      },
+      [&](int kc) {
+        rcv = rr[N + kc];
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+          const int kr = col_j() + 16 * e;
+          rrv[e] = rr[kr];
+          pkv[e] = pk[kr * N + kc];
+        }
+      });
   if (p.flags & QEM_FLAG_GRAD_POSITIONS) {
     const double tr = block_sum<T>(ar, sm.red);
     const double tc = block_sum<T>(ac, sm.red);
