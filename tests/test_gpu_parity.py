@@ -172,22 +172,31 @@ def test_depth_20_slices_256px_against_oracle_and_finite_differences(cuda):
         I = po.forward(o, p, case["positions"], props)["I"]
         return po.batch_loss(I, case["targets"], case["mask"], case["num_gpts"], case["mean_intensity"])
 
+    # The 32 + 32 directional derivatives are compared as vectors: their relative L2 error stays at the
+    # level of float32's own departure, while a sign or scale slip on any subset of them, small-gradient
+    # entries included, moves it by the size of that subset's share.  The oracle's own gradient is held
+    # to the finite differences far more tightly (float64 throughout).
     h = 1e-4
     rows, cols = po.patch_rows_cols(case["positions"], 256, obj.shape[1:])
-    gmax_o = np.abs(got["obj"]).max()
-    gmax_p = np.abs(got["probe"]).max()
+    fd_o, k_o, r_o, fd_p, k_p, r_p = [], [], [], [], [], []
     for t in range(32):
         s = rng.integers(0, 20)
         i, j = rows[0, rng.integers(96, 160)], cols[0, rng.integers(96, 160)]
         e = np.zeros_like(obj)
         e[s, i, j] = h
-        fd = (L(o=obj + e) - L(o=obj - e)) / (2 * h)
-        assert abs(fd - got["obj"][s, i, j].real) < 1e-3 * gmax_o + 1e-4 * abs(fd)
+        fd_o.append((L(o=obj + e) - L(o=obj - e)) / (2 * h))
+        k_o.append(got["obj"][s, i, j].real)
+        r_o.append(ref["obj"][s, i, j].real)
         mi, a, c = rng.integers(0, 2), rng.integers(0, 256), rng.integers(0, 256)
         e = np.zeros_like(probe)
         e[mi, a, c] = 1j * h
-        fd = (L(p=probe + e) - L(p=probe - e)) / (2 * h)
-        assert abs(fd - got["probe"][mi, a, c].imag) < 1e-3 * gmax_p + 1e-4 * abs(fd)
+        fd_p.append((L(p=probe + e) - L(p=probe - e)) / (2 * h))
+        k_p.append(got["probe"][mi, a, c].imag)
+        r_p.append(ref["probe"][mi, a, c].imag)
+    print(f"  finite differences: object kernel {rel(k_o, fd_o):.2e} oracle {rel(r_o, fd_o):.2e}; "
+          f"probe kernel {rel(k_p, fd_p):.2e} oracle {rel(r_p, fd_p):.2e}", flush=True)
+    assert rel(r_o, fd_o) < 1e-6 and rel(r_p, fd_p) < 1e-6
+    assert rel(k_o, fd_o) < 5e-4 and rel(k_p, fd_p) < 5e-4
 
 
 def test_deterministic(cuda):
