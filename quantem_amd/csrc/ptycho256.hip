@@ -254,7 +254,10 @@ __device__ __forceinline__ void col_pass(const cf* src, cf* dst, SmemOne& sm, Op
     for (int e = 0; e < 16; ++e) v[e] = dma_col_value(sm, e);
     lds_barrier();  // every wave has read the chunk before the next copy overwrites it
     pre(c);
+    bool issued = false;  // an op that never calls next() still gets the copy, after it returns
     auto next = [&] {
+      if (issued) return;
+      issued = true;
       __builtin_amdgcn_sched_barrier(0);  // op's use of its loads stays before the copy
       if (c0 + CH < N) dma_cols(src, c0 + CH, sm);
       __builtin_amdgcn_sched_barrier(0);  // and the copy is issued before what follows
@@ -263,8 +266,8 @@ __device__ __forceinline__ void col_pass(const cf* src, cf* dst, SmemOne& sm, Op
       op(c, v, next);
     } else {
       op(c, v);
-      next();
     }
+    next();
 #pragma unroll
     for (int e = 0; e < 16; ++e) dst[(j + 16 * e) * N + c] = v[e];
   }
