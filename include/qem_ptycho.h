@@ -62,9 +62,19 @@ enum {
   QEM_FLAG_GRAD_POSITIONS = 1u << 1, /* adjoint also writes d loss / d positions */
   QEM_FLAG_GRAD_DESCAN = 1u << 2,    /* adjoint also writes d loss / d descan */
   QEM_FLAG_ANALYTIC = 1u << 3,       /* set by qem_ptycho_projection_update (internal) */
-  QEM_FLAG_GRAD_PROPAGATORS = 1u << 4 /* with QEM_FLAG_STORE_WAVES and segment_slices 0: keep what
+  QEM_FLAG_GRAD_PROPAGATORS = 1u << 4, /* with QEM_FLAG_STORE_WAVES and segment_slices 0: keep what
                                          qem_ptycho_propagator_gradient needs (one more (S-1)-slice
                                          spectrum per pattern and mode in the workspace) */
+  QEM_FLAG_FULL_PROPAGATORS = 1u << 5  /* 256 x 256 patterns: apply every propagator entry as given.
+                                         By default a propagator slice that is the product of a row
+                                         and a column factor (the reference's Fresnel propagators,
+                                         tilted or not: exp(-i pi lambda dz (kr^2 + kc^2)) times the
+                                         tilt's per-axis terms) is applied as that product, the
+                                         factors taken from its first column and first row, when for
+                                         every slice the product reproduces each entry within 1e-6 of
+                                         the largest magnitude among its 32 rows (checked on the GPU by
+                                         qem_ptycho_forward_loss, kept in the workspace for the
+                                         adjoint); otherwise every slice is applied entry by entry. */
 };
 
 typedef struct qem_ptycho_desc {
@@ -114,7 +124,7 @@ int qem_ptycho_forward_loss(const qem_ptycho_desc* desc, const void* obj, const 
 
 /* qem_ptycho_forward_loss without a workspace argument, for callers that cannot allocate device memory
  * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
- * counterpart in the reference.  The no-gradient forward only (desc->flags must be 0: nothing is kept
+ * counterpart in the reference.  The no-gradient forward only (desc->flags must be 0 or QEM_FLAG_FULL_PROPAGATORS: nothing is kept
  * for an adjoint); its workspace is a file-scope device array of QEM_STATIC_WORKSPACE_BYTES bytes that
  * each GPU holds from the first call on it until the process ends.  Requests whose workspace
  * (qem_ptycho_workspace_bytes with flags 0) exceeds it return QEM_ERR_WORKSPACE: at most 111 patterns

@@ -25,6 +25,7 @@ QEM_FLAG_GRAD_POSITIONS = 1 << 1
 QEM_FLAG_GRAD_DESCAN = 1 << 2
 QEM_FLAG_ANALYTIC = 1 << 3
 QEM_FLAG_GRAD_PROPAGATORS = 1 << 4
+QEM_FLAG_FULL_PROPAGATORS = 1 << 5
 
 QEM_STATIC_WORKSPACE_BYTES = 64 << 20  # workspace of qem_ptycho_forward_loss_static
 

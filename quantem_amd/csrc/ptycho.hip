@@ -1024,7 +1024,8 @@ void prof_mark(bool adjoint_kernel, int edge, hipStream_t st) {
 }
 
 struct Layout {
-  size_t psi, ckpt, scratch, X, gI, loss_part, pk, gsum, gpos, gdesc, ramps, pspec, total;
+  size_t psi, ckpt, scratch, X, gI, loss_part, pk, gsum, gpos, gdesc, ramps, pspec, pfac, pfac_ok, total;
+  bool pfactors;  // separable-propagator factors checked and kept (N = 256, S > 1, no QEM_FLAG_FULL_PROPAGATORS)
   bool pgrad;  // propagator-gradient spectra kept (QEM_FLAG_GRAD_PROPAGATORS with stored waves)
   int nloss;
   int wslots;  // wave slots per item in psi
@@ -1075,6 +1076,11 @@ Layout layout_of(const qem_ptycho_desc& d) {
   L.pgrad = store && (d.flags & QEM_FLAG_GRAD_PROPAGATORS) && S > 1;
   L.pspec = o;
   if (L.pgrad) o += align_up(BM * (size_t)(S - 1) * N2 * 8);
+  L.pfactors = fixed_size(d) && d.roi == 256 && S > 1 && !(d.flags & QEM_FLAG_FULL_PROPAGATORS);
+  L.pfac = o;
+  if (L.pfactors) o += align_up((size_t)(S - 1) * 2 * d.roi * 8);
+  L.pfac_ok = o;
+  if (L.pfactors) o += align_up((size_t)(S - 1) * 8 * 4);  // one verdict per 32 rows of a slice
   L.total = o;
   return L;
 }
@@ -1165,6 +1171,8 @@ qem::Params make_params(const qem_ptycho_desc& d, const Layout& L, void* ws) {
   p.gpos_part = (float*)(w + L.gpos);
   p.gdesc_part = (float*)(w + L.gdesc);
   p.pspec = L.pgrad ? (qem::cf*)(w + L.pspec) : nullptr;
+  p.pfac = L.pfactors ? (qem::cf*)(w + L.pfac) : nullptr;
+  p.pfac_ok = L.pfactors ? (int*)(w + L.pfac_ok) : nullptr;
   return p;
 }
 
@@ -1519,8 +1527,10 @@ int qem_ptycho_forward_loss_static(const qem_ptycho_desc* desc, const void* obj,
                                    void* stream) {
   qem::clear_error();
   if (int rc = check_desc(desc)) return rc;
-  if (desc->flags != 0)
-    return fail(QEM_ERR_ARG, "the static-workspace forward runs the no-gradient forward only (flags must be 0)");
+  if (desc->flags & ~(uint32_t)QEM_FLAG_FULL_PROPAGATORS)
+    return fail(QEM_ERR_ARG,
+                "the static-workspace forward runs the no-gradient forward only (flags must be 0 or "
+                "QEM_FLAG_FULL_PROPAGATORS)");
   const size_t need = layout_of(*desc).total;
   if (need > (size_t)QEM_STATIC_WORKSPACE_BYTES)
     return fail(QEM_ERR_WORKSPACE,

@@ -49,6 +49,7 @@ struct Smem {
 // transformed; the footprint above half a CU's 160 KiB also keeps one workgroup per CU.
 struct SmemOne : Smem {
   cf dma[CH * N];  // the next chunk of W: rows [e][vector][j] or columns [e][j][column] (see dma_rows / dma_cols)
+  cf prow[N];      // row factor a[kr] of a separable propagator slice (see k_prop_factors256)
 };
 static_assert(sizeof(SmemOne) > 80 * 1024 && sizeof(SmemOne) <= 160 * 1024, "one workgroup per CU");
 
@@ -322,9 +323,87 @@ __global__ __launch_bounds__(T) void k_spectrum256(Params p) {
   col_pass(Wb, p.pk + (size_t)m * N2, sm, [&](int, cf(&v)[16]) { fft256<false, true>(v, sm); });
 }
 
+// Separable propagator slices.  The reference's Fresnel propagator exp(-i pi lambda dz (kr^2 + kc^2)),
+// with or without the tilt's per-axis phase terms (probe_models.py:376-427), is a[kr] b[kc]; applied as
+// that product, a column pass reads a 2 KiB row factor into LDS once and one column factor per lane
+// and chunk instead of its 64 KiB of propagator entries.  Block (s, part) checks rows
+// [32 part, 32 part + 32) of slice s: a[kr] = P[kr][0] and b[kc] = P[0][kc] / P[0][0] (quotient in
+// double, rounded once), a[kr] b[kc] formed in float as the kernels do, and the part marked separable
+// when none of its entries differs from it by more than 1e-6 of the largest magnitude among them (a
+// non-finite entry, or P[0][0] = 0, marks it not separable).  1024 threads: thread t reads column
+// t % 256 of every fourth row of the part from t / 256.
+constexpr int kFacT = 1024, kFacParts = 8, kFacRows = N / kFacParts;
+__global__ __launch_bounds__(kFacT) void k_prop_factors256(Params p) {
+  __shared__ cf a_s[kFacRows];
+  __shared__ double red[2][kFacT / 64];
+  const int s = blockIdx.x, part = blockIdx.y, t = threadIdx.x, c = t & (N - 1), g = t / N;
+  const int r0 = part * kFacRows;
+  const cf* P = p.prop + (size_t)s * N2;
+  cf* fa = p.pfac + (size_t)s * 2 * N;
+  const cf p00 = P[0], p0c = P[c];
+  const double d = (double)p00.x * p00.x + (double)p00.y * p00.y;
+  const cf b_c = d > 0.0 ? mk((float)(((double)p0c.x * p00.x + (double)p0c.y * p00.y) / d),
+                              (float)(((double)p0c.y * p00.x - (double)p0c.x * p00.y) / d))
+                         : mk(0.0f, 0.0f);
+  if (t < kFacRows) {
+    const cf a_r = P[(size_t)(r0 + t) * N];
+    a_s[t] = a_r;
+    fa[r0 + t] = a_r;
+  }
+  if (part == 0 && g == 0) fa[N + c] = b_c;
+  __syncthreads();
+  constexpr int G = kFacT / N, RPT = kFacRows / G;  // row groups, rows per thread
+  cf pv[RPT];
+#pragma unroll
+  for (int i = 0; i < RPT; ++i) pv[i] = P[(size_t)(r0 + g + G * i) * N + c];
+  double dev = 0.0, mag = 0.0;
+#pragma unroll
+  for (int i = 0; i < RPT; ++i) {
+    const cf q = cmul(a_s[g + G * i], b_c);
+    const double ex = (double)q.x - (double)pv[i].x, ey = (double)q.y - (double)pv[i].y;
+    const double e = ex * ex + ey * ey, m = (double)pv[i].x * pv[i].x + (double)pv[i].y * pv[i].y;
+    if (!(e <= dev)) dev = e;  // a NaN sticks
+    if (!(m <= mag)) mag = m;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double od = __shfl_down(dev, o, 64), om = __shfl_down(mag, o, 64);
+    if (!(od <= dev)) dev = od;
+    if (!(om <= mag)) mag = om;
+  }
+  if ((t & 63) == 0) {
+    red[0][t >> 6] = dev;
+    red[1][t >> 6] = mag;
+  }
+  __syncthreads();
+  if (t == 0) {
+    for (int w = 1; w < kFacT / 64; ++w) {
+      if (!(red[0][w] <= dev)) dev = red[0][w];
+      if (!(red[1][w] <= mag)) mag = red[1][w];
+    }
+    // squared magnitudes: deviation <= 1e-6 * largest magnitude of the part
+    p.pfac_ok[s * kFacParts + part] = (d > 0.0 && dev <= 1e-12 * mag) ? 1 : 0;
+  }
+}
+
+// Whether every propagator slice of the launch is separable (k_prop_factors256's verdicts).  The
+// multislice kernels come in two instantiations, FAC (every slice through its factors) and not (every
+// entry as given, as when p.pfac is null); with factors checked, both are launched and the one that
+// does not match the verdicts returns at once, so the host never waits for them.
+__device__ __forceinline__ bool all_separable(const Params& p) {
+  if (p.pfac == nullptr) return false;
+  const int n = (p.S - 1) * kFacParts;
+  bool all = true;
+  for (int i0 = 0; i0 < n; i0 += 64) {
+    const int i = i0 + (int)(threadIdx.x & 63);
+    all = all && __ballot(i < n && p.pfac_ok[i] == 0) == 0ull;
+  }
+  return all;
+}
+
 // Forward multislice of one (pattern, mode); the modes (p.fwd_mode) are those of k_forward<N>.
-template <bool DESCAN, bool PG = false>
+template <bool DESCAN, bool PG = false, bool FAC = false>
 __global__ __launch_bounds__(T) void k_forward256(Params p) {
+  if (all_separable(p) != FAC) return;  // the other instantiation runs this launch
   __shared__ SmemOne sm;
   init_tw(sm);
   const int item = xcd_item(blockIdx.x, gridDim.x);
@@ -431,7 +510,12 @@ __global__ __launch_bounds__(T) void k_forward256(Params p) {
     if (s + 1 < p.s1) {
       const cf* pr = p.prop + (size_t)s * N2;
       cf* xs = PG ? p.pspec + ((size_t)item * (p.S - 1) + s) * N2 : nullptr;
-      cf pq[16];
+      cf pq[FAC ? 1 : 16];
+      cf pb;
+      if constexpr (FAC) {
+        // separable: a[kr] from LDS (read after the pass's first barrier) times this lane's b[kc] / N^2
+        if (threadIdx.x < N) sm.prow[threadIdx.x] = p.pfac[(size_t)s * 2 * N + threadIdx.x];
+      }
       col_pass(
           Wb, Wb, sm,
           [&](int kc, cf(&v)[16], auto&& next) {
@@ -441,14 +525,23 @@ __global__ __launch_bounds__(T) void k_forward256(Params p) {
 #pragma unroll
               for (int e = 0; e < 16; ++e) st_stream(xs + (jj + 16 * e) * N + kc, v[e]);
             }
+            if constexpr (FAC) {
 #pragma unroll
-            for (int e = 0; e < 16; ++e) v[e] = cscale(cmul(v[e], pq[e]), inv_n2);
+              for (int e = 0; e < 16; ++e) v[e] = cmul(v[e], cmul(sm.prow[jj + 16 * e], pb));
+            } else {
+#pragma unroll
+              for (int e = 0; e < 16; ++e) v[e] = cscale(cmul(v[e], pq[e]), inv_n2);
+            }
             next();
             fft256<true, true>(v, sm);
           },
           [&](int kc) {
+            if constexpr (FAC) {
+              pb = cscale(p.pfac[(size_t)s * 2 * N + N + kc], inv_n2);
+            } else {
 #pragma unroll
-            for (int e = 0; e < 16; ++e) pq[e] = pr[(col_j() + 16 * e) * N + kc];
+              for (int e = 0; e < 16; ++e) pq[e] = pr[(col_j() + 16 * e) * N + kc];
+            }
           });
     } else if (far) {
       const float inv_n = 1.0f / (float)N;
@@ -466,8 +559,9 @@ __global__ __launch_bounds__(T) void k_forward256(Params p) {
 // starts from the far field; a lower one from the row-transformed g that the segment above left in X.
 // The bottom segment ends with G = fft2(g_psi_0): probe-gradient spectrum conj(ramp) G over X and the
 // position-gradient partial sums; any other segment leaves its row-transformed g in X.
-template <bool DESCAN, bool PG = false>
+template <bool DESCAN, bool PG = false, bool FAC = false>
 __global__ __launch_bounds__(T) void k_adjoint256(Params p) {
+  if (all_separable(p) != FAC) return;  // the other instantiation runs this launch
   __shared__ SmemOne sm;
   init_tw(sm);
   const int item = xcd_item(blockIdx.x, gridDim.x);
@@ -568,11 +662,14 @@ __global__ __launch_bounds__(T) void k_adjoint256(Params p) {
     if (s > 0) {
       // g_x <- ifft2(fft2(g_psi) conj(prop_{s-1})); at a segment bottom it goes to X for the segment below
       const cf* pr = p.prop + (size_t)(s - 1) * N2;
-      // (the propagator column is loaded before the column DFT, which hides its latency: measured
-      // faster here, slower in the forward's column pass)
       cf* xs = PG ? p.pspec + ((size_t)item * (p.S - 1) + (s - 1)) * N2 : nullptr;
-      cf q[16];
+      cf q[FAC ? 1 : 16];
+      cf qb;
       cf xq[PG ? 16 : 1];
+      if constexpr (FAC) {
+        // separable: conj(a[kr] b[kc] / N^2), a[kr] from LDS (read after the pass's first barrier)
+        if (threadIdx.x < N) sm.prow[threadIdx.x] = p.pfac[(size_t)(s - 1) * 2 * N + threadIdx.x];
+      }
       col_pass(
           Wb, s == p.s0 ? Xs : Wb, sm,
           [&](int kc, cf(&v)[16], auto&& next) {
@@ -582,14 +679,23 @@ __global__ __launch_bounds__(T) void k_adjoint256(Params p) {
 #pragma unroll
           for (int e = 0; e < 16; ++e) st_stream(xs + (jj + 16 * e) * N + kc, cscale(cconjmul(xq[e], v[e]), inv_n2));
         }
+        if constexpr (FAC) {
 #pragma unroll
-        for (int e = 0; e < 16; ++e) v[e] = cscale(cmulc(v[e], q[e]), inv_n2);
+          for (int e = 0; e < 16; ++e) v[e] = cmulc(v[e], cmul(sm.prow[jj + 16 * e], qb));
+        } else {
+#pragma unroll
+          for (int e = 0; e < 16; ++e) v[e] = cscale(cmulc(v[e], q[e]), inv_n2);
+        }
         next();
         fft256<true, true>(v, sm);
           },
           [&](int kc) {
+            if constexpr (FAC) {
+              qb = cscale(p.pfac[(size_t)(s - 1) * 2 * N + N + kc], inv_n2);
+            } else {
 #pragma unroll
-            for (int e = 0; e < 16; ++e) q[e] = pr[(col_j() + 16 * e) * N + kc];
+              for (int e = 0; e < 16; ++e) q[e] = pr[(col_j() + 16 * e) * N + kc];
+            }
             if constexpr (PG) {
 #pragma unroll
               for (int e = 0; e < 16; ++e) xq[e] = ld_stream(xs + (col_j() + 16 * e) * N + kc);
@@ -726,6 +832,11 @@ __global__ __launch_bounds__(T) void k_center256(int P, const float* __restrict_
 unsigned long long params_signature_256() { return params_signature_local(); }
 
 int launch_chain256_forward(const Params& p, hipStream_t st) {
+  if (p.pfac != nullptr && p.S > 1) {
+    // the factors and verdicts stay in the workspace for the adjoint and its recomputed segments
+    hipLaunchKernelGGL(n256::k_prop_factors256, dim3(p.S - 1, n256::kFacParts), dim3(n256::kFacT), 0, st, p);
+    if (int rc = check_last_launch("prop_factors256")) return rc;
+  }
   // M probe-spectrum blocks, then the blocks of the (B, 2, N) shift-ramp table
   const int ramp_blocks = (p.B * 2 * n256::N + n256::T - 1) / n256::T;
   hipLaunchKernelGGL(n256::k_spectrum256, dim3(p.M + ramp_blocks), dim3(n256::T), 0, st, p);
@@ -733,27 +844,43 @@ int launch_chain256_forward(const Params& p, hipStream_t st) {
   return launch_forward256_kernel(p, st);
 }
 
+// With checked factors (p.pfac) both instantiations are launched: the FAC one runs when every slice is
+// separable, the other otherwise, and the one that does not run returns at its first instruction.
+template <bool DESCAN, bool PG>
+static void launch_fwd256(const Params& p, hipStream_t st) {
+  if (p.pfac != nullptr)
+    hipLaunchKernelGGL((n256::k_forward256<DESCAN, PG, true>), dim3(p.B * p.M), dim3(n256::T), 0, st, p);
+  hipLaunchKernelGGL((n256::k_forward256<DESCAN, PG, false>), dim3(p.B * p.M), dim3(n256::T), 0, st, p);
+}
+
+template <bool DESCAN, bool PG>
+static void launch_adj256(const Params& p, hipStream_t st) {
+  if (p.pfac != nullptr)
+    hipLaunchKernelGGL((n256::k_adjoint256<DESCAN, PG, true>), dim3(p.B * p.M), dim3(n256::T), 0, st, p);
+  hipLaunchKernelGGL((n256::k_adjoint256<DESCAN, PG, false>), dim3(p.B * p.M), dim3(n256::T), 0, st, p);
+}
+
 int launch_forward256_kernel(const Params& p, hipStream_t st) {
   if (p.pspec != nullptr && p.descan != nullptr)
-    hipLaunchKernelGGL((n256::k_forward256<true, true>), dim3(p.B * p.M), dim3(n256::T), 0, st, p);
+    launch_fwd256<true, true>(p, st);
   else if (p.pspec != nullptr)
-    hipLaunchKernelGGL((n256::k_forward256<false, true>), dim3(p.B * p.M), dim3(n256::T), 0, st, p);
+    launch_fwd256<false, true>(p, st);
   else if (p.descan != nullptr && p.fwd_mode != FWD_RECOMPUTE)
-    hipLaunchKernelGGL((n256::k_forward256<true>), dim3(p.B * p.M), dim3(n256::T), 0, st, p);
+    launch_fwd256<true, false>(p, st);
   else
-    hipLaunchKernelGGL((n256::k_forward256<false>), dim3(p.B * p.M), dim3(n256::T), 0, st, p);
+    launch_fwd256<false, false>(p, st);
   return check_last_launch("forward256");
 }
 
 int launch_chain256_adjoint(const Params& p, hipStream_t st) {
   if (p.pspec != nullptr && p.descan != nullptr)
-    hipLaunchKernelGGL((n256::k_adjoint256<true, true>), dim3(p.B * p.M), dim3(n256::T), 0, st, p);
+    launch_adj256<true, true>(p, st);
   else if (p.pspec != nullptr)
-    hipLaunchKernelGGL((n256::k_adjoint256<false, true>), dim3(p.B * p.M), dim3(n256::T), 0, st, p);
+    launch_adj256<false, true>(p, st);
   else if (p.descan != nullptr)
-    hipLaunchKernelGGL((n256::k_adjoint256<true>), dim3(p.B * p.M), dim3(n256::T), 0, st, p);
+    launch_adj256<true, false>(p, st);
   else
-    hipLaunchKernelGGL((n256::k_adjoint256<false>), dim3(p.B * p.M), dim3(n256::T), 0, st, p);
+    launch_adj256<false, false>(p, st);
   return check_last_launch("adjoint256");
 }
 

@@ -56,6 +56,11 @@ struct Params {
   // wave (P3 layout, slots s = 0..S-2 per item); the adjoint overwrites each with that item's
   // contribution conj(X) fft2(g_psi) / N^2 to dL/dP_s, which a fixed-order reduce sums over the items
   cf* pspec;     // (B*M*(S-1), N, N) or nullptr
+  // separable propagators (N = 256 chain): per slice its row factor a[kr] = prop[kr][0] and column
+  // factor b[kc] = prop[0][kc] / prop[0][0] ((S-1, 2, N)), and whether a[kr] b[kc] reproduces each
+  // 32-row part of the slice ((S-1, 8) ints, 1 = yes); nullptr = apply every propagator entry as given
+  cf* pfac;
+  int* pfac_ok;
 };
 
 // Layout signature of Params as compiled in the current translation unit: its size and every field's
@@ -68,7 +73,8 @@ struct Params {
   QF(S) QF(M) QF(B) QF(H) QF(W) QF(P) QF(loss_type) QF(NR) QF(NC) QF(mean_I) QF(flags) QF(obj) QF(probe) QF(pos) QF(prop)        \
   QF(targets) QF(mask) QF(descan) QF(loss_out) QF(intens_out) QF(dloss) QF(grad_obj) QF(grad_probe)           \
   QF(grad_pos) QF(grad_descan) QF(psi) QF(scratch) QF(X) QF(gI) QF(loss_part) QF(pk) QF(gsum) QF(ramps)          \
-  QF(gpos_part) QF(gdesc_part) QF(s0) QF(s1) QF(wslots) QF(ck) QF(nck) QF(fwd_mode) QF(ckpt) QF(pspec)
+  QF(gpos_part) QF(gdesc_part) QF(s0) QF(s1) QF(wslots) QF(ck) QF(nck) QF(fwd_mode) QF(ckpt) QF(pspec)  \
+  QF(pfac) QF(pfac_ok)
 // internal linkage: each translation unit evaluates its own copy (an inline function would be merged)
 static constexpr unsigned long long params_signature_local() {
   unsigned long long h = sizeof(Params);

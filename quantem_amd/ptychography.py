@@ -114,7 +114,7 @@ class MultisliceLoss(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, obj, probe, positions, propagators, targets, mask, descan, num_gpts, mean_intensity, loss_type,
-                return_intensities, grad_enabled, segment_slices):
+                return_intensities, grad_enabled, segment_slices, full_propagators=False):
         device = obj.device
         if device.type != "cuda":
             raise ValueError(f"obj must be on a GPU device, got {device}")
@@ -150,7 +150,7 @@ class MultisliceLoss(torch.autograd.Function):
         # grad mode is always off inside Function.forward: the caller's mode arrives as grad_enabled
         grad_prop = bool(grad_enabled) and prop_c is not None and ctx.needs_input_grad[3]
         need_grad = bool(grad_enabled) and (any(ctx.needs_input_grad[i] for i in (0, 1, 2, 6)) or grad_prop)
-        flags = 0
+        flags = nat.QEM_FLAG_FULL_PROPAGATORS if full_propagators else 0
         if need_grad:
             flags |= nat.QEM_FLAG_STORE_WAVES
             if ctx.needs_input_grad[2]:
@@ -208,7 +208,7 @@ class MultisliceLoss(torch.autograd.Function):
             nat.check(rc, "qem_ptycho_propagator_gradient")
         return (g_obj if ctx.needs_input_grad[0] else None,
                 g_probe if ctx.needs_input_grad[1] else None,
-                g_pos, g_prop, None, None, g_desc, None, None, None, None, None, None)
+                g_pos, g_prop, None, None, g_desc, None, None, None, None, None, None, None)
 
 
 def multislice_loss(
@@ -225,6 +225,7 @@ def multislice_loss(
     loss_type: str = "l2_amplitude",
     return_intensities: bool = False,
     segment_slices: int = 0,
+    full_propagators: bool = False,
 ):
     """Batch loss of quantem's forward model (``ptychography.py:220-229``), differentiable.
 
@@ -246,6 +247,10 @@ def multislice_loss(
         segment_slices: 0 keeps every slice's incident wave for the backward pass; k in [2, S) keeps
             only every k-th one and recomputes the others in the backward pass, one k-slice segment
             at a time (less memory, one extra propagation per slice).
+        full_propagators: 256 x 256 patterns apply a propagator slice that factors into a row and a
+            column term (the reference's Fresnel propagators, tilted or not) as that product, after
+            checking on the GPU that it reproduces every entry within 1e-6 of the largest; True
+            applies every entry as given (``QEM_FLAG_FULL_PROPAGATORS``).
 
     Returns:
         loss (0-d float32 tensor), and intensities if requested.
@@ -256,7 +261,7 @@ def multislice_loss(
         raise ValueError(f"segment_slices must be >= 0, got {segment_slices}")
     loss, intens = MultisliceLoss.apply(obj, probe, positions, propagators, targets, mask, descan, num_gpts,
                                         mean_intensity, loss_type, return_intensities, torch.is_grad_enabled(),
-                                        int(segment_slices))
+                                        int(segment_slices), bool(full_propagators))
     return (loss, intens) if return_intensities else loss
 
 

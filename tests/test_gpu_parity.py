@@ -26,7 +26,7 @@ def _t(x, dev):
     return None if x is None else torch.from_numpy(np.ascontiguousarray(x)).to(dev)
 
 
-def _run(case, dev, grads=True, pos_grad=True, desc_grad=True, segment_slices=0):
+def _run(case, dev, grads=True, pos_grad=True, desc_grad=True, segment_slices=0, full_propagators=False):
     obj = _t(case["obj"], dev).requires_grad_(grads)
     probe = _t(case["probe"], dev).requires_grad_(grads)
     pos = _t(case["positions"], dev).requires_grad_(grads and pos_grad)
@@ -37,7 +37,7 @@ def _run(case, dev, grads=True, pos_grad=True, desc_grad=True, segment_slices=0)
     loss, intens = qp.multislice_loss(obj, probe, pos, props, _t(case["targets"], dev), _t(case["mask"], dev), d,
                                       num_gpts=case["num_gpts"], mean_intensity=case["mean_intensity"],
                                       loss_type=case["loss_type"], return_intensities=True,
-                                      segment_slices=segment_slices)
+                                      segment_slices=segment_slices, full_propagators=full_propagators)
     out = {"loss": loss.item(), "I": intens.cpu().numpy()}
     if grads:
         loss.backward()
@@ -129,6 +129,53 @@ def test_segment_recompute_matches_stored_and_oracle(cuda, n, s, m, b, k, descan
     for key in ("obj", "probe", "positions") + (("descan",) if descan else ()):
         assert rel(seg[key], full[key]) < 1e-6, key
         assert rel(seg[key], ref[key]) < GRAD_TOL, key
+
+
+def test_separable_propagators_256(cuda):
+    """256 x 256 patterns apply a Fresnel propagator slice (a row factor times a column factor) through
+    its factors by default: the intensities stay within 1e-6 of every entry applied as given
+    (QEM_FLAG_FULL_PROPAGATORS) and differ from them in the last bits (the factor path ran), the
+    gradients within 2e-5 (they carry the residual |psi| - target, here about a tenth of |psi|, which
+    scales the float32 differences up), and both paths meet the oracle bar.  At 0.1 A sampling the largest propagator phase is 6 rad and the float32 entries factor
+    to 4e-7; the other 256 x 256 cases here sample at 2/256 A, where phases near 1000 rad leave 5e-5 of
+    float32 rounding that no product reproduces, so they run the entry-by-entry path.  A slice
+    multiplied by a random phase does not factor: one such slice sends the whole launch down the
+    entry-by-entry path (bitwise the results with QEM_FLAG_FULL_PROPAGATORS), where a factor applied to
+    the perturbed slice would miss the oracle by the size of the perturbation (about 1e-1)."""
+    print("separable propagators n=256 s=3 m=2 b=3", flush=True)
+    case = _case_with_targets(N=256, S=3, M=2, B=3, seed=41, descan=True, sampling=0.1)
+    fac = _run(case, cuda)
+    full = _run(case, cuda, full_propagators=True)
+    ref = _oracle(case)
+    keys = ("obj", "probe", "positions", "descan")
+    print("  factors vs entries: " + ", ".join(f"{k} {rel(fac[k], full[k]):.2e}" for k in ("I",) + keys), flush=True)
+    for name, got in (("factors", fac), ("entries", full)):
+        print(f"  {name} vs oracle: " + ", ".join(f"{k} {rel(got[k], ref[k]):.2e}" for k in ("I",) + keys), flush=True)
+    assert not np.array_equal(fac["I"], full["I"])
+    assert rel(fac["I"], full["I"]) < 1e-6
+    for k in keys:
+        assert rel(fac[k], full[k]) < 2e-5, k
+    for got in (fac, full):
+        assert rel(got["I"], ref["I"]) < FWD_TOL
+        for k in keys:
+            assert rel(got[k], ref[k]) < GRAD_TOL, k
+
+    print("one slice not separable", flush=True)
+    mixed = make_case(N=256, S=3, M=2, B=3, seed=42, sampling=0.1)
+    rng = np.random.default_rng(43)
+    mixed["propagators"][1] *= np.exp(0.3j * rng.uniform(-1.0, 1.0, (256, 256))).astype(np.complex64)
+    I = po.forward(mixed["obj"], mixed["probe"], mixed["positions"], mixed["propagators"])["I"]
+    mixed = attach_targets(mixed, I)
+    got = _run(mixed, cuda)
+    full = _run(mixed, cuda, full_propagators=True)
+    ref = _oracle(mixed)
+    errs = {k: rel(got[k], ref[k]) for k in ("I", "obj", "probe", "positions")}
+    print("  " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
+    assert errs["I"] < FWD_TOL
+    assert np.array_equal(got["I"], full["I"])
+    for k in ("obj", "probe", "positions"):
+        assert errs[k] < GRAD_TOL, k
+        assert np.array_equal(got[k], full[k]), k
 
 
 def test_depth_20_slices_256px_against_oracle_and_finite_differences(cuda):
