@@ -59,6 +59,15 @@ static_assert(sizeof(SmemOne) > 80 * 1024 && sizeof(SmemOne) <= 160 * 1024, "one
 // current one is transformed.  The waits are explicit: a pass waits with vmcnt(16) for its own copy,
 // allowing only the previous chunk's 16 row / column stores (the last vector-memory instructions of
 // the wave) to stay in flight; loads and stores share the one in-order vmcnt counter.
+// Orders this wave's LDS accesses: the lanes of one wave issue LDS instructions together and the LDS
+// serves one wave's instructions in order, so a read after this sees every write of the wave before
+// it; only the compiler has to be kept from moving accesses across (as rocPRIM's wave_barrier).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 typedef __attribute__((address_space(3))) void* lds_void_ptr;
 
 // One copy instruction: 16 bytes per lane from base + lane_off to the LDS block at lds (its 64 lanes'
@@ -72,23 +81,23 @@ __device__ __forceinline__ void lds_dma16(const void* base, unsigned lane_off, c
   asm volatile("s_nop 0\n\tglobal_load_lds_dwordx4 %0, off" : : "v"(addr), "{m0}"(m0) : "memory");
 }
 
-// Row chunk [r0, r0 + CH) of src into dma laid out [e][vector][j] (value j + 16 e of row r0 + vector):
-// a wave's read of its v[e] is then 512 contiguous bytes.  The wave index is made scalar, so that the
-// block offsets are scalar and each copy instruction is a scalar base plus one 32-bit lane offset.
+// Row chunk [r0, r0 + CH) of src into dma, each wave's four rows (vectors 4 w .. 4 w + 3) in its own
+// 8 KiB region laid out [e][row][j] (value j + 16 e of row r0 + 4 w + row): a wave copies and reads only
+// its own rows, so the row passes need no workgroup barrier, and a wave's read of its v[e] is 512
+// contiguous bytes.  The wave index is made scalar, so that the block offsets are scalar and each copy
+// instruction is a scalar base plus one 32-bit lane offset.
 __device__ __forceinline__ void dma_rows(const cf* src, int r0, SmemOne& sm) {
   const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int l = threadIdx.x & 63;
-  unsigned lane = (unsigned)((l >> 3) * N + 2 * (l & 7)) * 8u;  // row (l / 8) of the block, values 2 (l % 8), +1
+  // lane: e parity l / 32, row (l % 32) / 8, values 2 (l % 8), +1
+  unsigned lane = (unsigned)(((l & 31) >> 3) * N + 2 * (l & 7) + 16 * (l >> 5)) * 8u;
   asm volatile("" : "+v"(lane));  // formed here, not kept live through the transforms
+  const unsigned wb = (unsigned)((r0 + 4 * w) * N) * 8u;
 #pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    const int blk = w * 8 + i;  // e = blk / 4, vectors (blk % 4) * 8 .. + 7
-    const unsigned ub = (unsigned)((r0 + (blk & 3) * 8) * N + 16 * (blk >> 2)) * 8u;
-    lds_dma16(src, ub + lane, &sm.dma[blk * 128]);
-  }
+  for (int i = 0; i < 8; ++i) lds_dma16(src, wb + 256u * i + lane, &sm.dma[w * 1024 + i * 128]);  // e = 2 i, 2 i + 1
 }
 __device__ __forceinline__ cf dma_row_value(const SmemOne& sm, int e) {
-  return sm.dma[(e * CH + (int)(threadIdx.x >> 4)) * 16 + (int)(threadIdx.x & 15)];
+  return sm.dma[(int)(threadIdx.x >> 6) * 1024 + e * 64 + (int)(threadIdx.x & 63)];
 }
 // Column chunk [c0, c0 + CH) of src into dma laid out [e][j][column ^ 16 (j & 1)] (value j + 16 e of
 // column c0 + column): the swizzle spreads a wave's four j rows of 128 bytes over both halves of the
@@ -111,16 +120,21 @@ __device__ __forceinline__ cf dma_col_value(const SmemOne& sm, int e) {
   const int c = (threadIdx.x & 15) + 16 * (threadIdx.x >> 8);
   return sm.dma[((e * 16 + j) * CH) + (c ^ (16 * (j & 1)))];
 }
-// Waits for this wave's part of the chunk copy, then for every wave's (barrier).  `stores` = the
-// number of vector-memory instructions issued after the copy (the previous chunk's stores), which may
-// stay in flight: vmcnt(16) or vmcnt(0).
+// Waits for this wave's copies (`stores` = the number of vector-memory instructions issued after them,
+// the previous chunk's stores, which may stay in flight: vmcnt(16) or vmcnt(0)), then, for a chunk
+// that other waves copied (columns), for every wave's (barrier).
+template <bool SHARED>
 __device__ __forceinline__ void dma_wait(bool stores) {
   if (stores)
     __builtin_amdgcn_s_waitcnt(0x4F70);  // vmcnt(16)
   else
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
-  __builtin_amdgcn_s_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+  if constexpr (SHARED) {
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+  } else {
+    wave_lds_sync();
+  }
 }
 
 // Lane roles.  Row passes: vector (row) t / 16, j = t % 16.  Column passes: column t % 16 + 16 (t / 256)
@@ -145,7 +159,9 @@ __device__ __forceinline__ int ex_idx(int j, int k1) {
 }
 
 // In-place length-256 DFT (unnormalised; conjugate twiddles if INV) of this lane's vector:
-// on entry v[i] = x[j + 16 i], on return v[k2] = X[j + 16 k2].
+// on entry v[i] = x[j + 16 i], on return v[k2] = X[j + 16 k2].  A row's 16 lanes are 16 consecutive
+// lanes of one wave and its exchange block is its own, so the row exchange needs no workgroup barrier;
+// a column's 16 lanes lie in four waves.
 template <bool INV, bool COLS>
 __device__ __forceinline__ void fft256(cf (&v)[16], Smem& sm) {
   const int j = COLS ? col_j() : row_j();
@@ -155,10 +171,16 @@ __device__ __forceinline__ void fft256(cf (&v)[16], Smem& sm) {
     const cf w = sm.tw[(j * k1) & 255];
     v[k1] = INV ? cmulc(v[k1], w) : cmul(v[k1], w);
   }
-  lds_barrier();  // earlier readers of sm.ex are done
+  if constexpr (COLS)
+    lds_barrier();  // earlier readers of sm.ex are done
+  else
+    wave_lds_sync();
 #pragma unroll
   for (int k1 = 0; k1 < 16; ++k1) sm.ex[ex_idx<COLS>(j, k1)] = v[k1];
-  lds_barrier();
+  if constexpr (COLS)
+    lds_barrier();
+  else
+    wave_lds_sync();
 #pragma unroll
   for (int jj = 0; jj < 16; ++jj) v[jj] = sm.ex[ex_idx<COLS>(jj, j)];  // this lane now owns k1 = j
   Dft<16, INV>::template run<1>(v);                                     // over jj -> k2
@@ -221,11 +243,11 @@ __device__ __forceinline__ void row_pass(const cf* src, cf* dst, SmemOne& sm, Op
   dma_rows(src, 0, sm);
   for (int r0 = 0; r0 < N; r0 += CH) {
     const int r = r0 + row_vec();
-    dma_wait(r0 > 0 && dst != nullptr);
+    dma_wait<false>(r0 > 0 && dst != nullptr);
     cf v[16];
 #pragma unroll
     for (int e = 0; e < 16; ++e) v[e] = dma_row_value(sm, e);
-    lds_barrier();  // every wave has read the chunk before the next copy overwrites it
+    wave_lds_sync();  // the wave's reads stay before its next copy into the same region
     pre(r);
     if constexpr (PRE_INV) fft256<true, false>(v, sm);
     op(r, v);
@@ -249,7 +271,7 @@ __device__ __forceinline__ void col_pass(const cf* src, cf* dst, SmemOne& sm, Op
   dma_cols(src, 0, sm);
   for (int c0 = 0; c0 < N; c0 += CH) {
     const int c = c0 + col_vec();
-    dma_wait(c0 > 0);
+    dma_wait<true>(c0 > 0);
     cf v[16];
 #pragma unroll
     for (int e = 0; e < 16; ++e) v[e] = dma_col_value(sm, e);

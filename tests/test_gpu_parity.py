@@ -280,11 +280,13 @@ def test_argument_errors(cuda):
         qp.multislice_loss(obj, probe, pos, props, tg, num_gpts=8, mean_intensity=0.0)
 
 
-@pytest.mark.parametrize("n,s,m,b", [(32, 1, 1, 5), (64, 3, 2, 4), (128, 4, 3, 3), (256, 3, 2, 2), (256, 1, 1, 2)])
-def test_analytical_update_matches_oracle(cuda, n, s, m, b):
-    """The reference's autograd=False update (Fourier projection + slice replay)."""
-    print(f"analytical n={n} s={s} m={m}", flush=True)
-    case = _case_with_targets(N=n, S=s, M=m, B=b, seed=300 + n, descan=False)
+@pytest.mark.parametrize("n,s,m,b,samp", [(32, 1, 1, 5, None), (64, 3, 2, 4, None), (128, 4, 3, 3, None),
+                                          (256, 3, 2, 2, None), (256, 3, 2, 2, 0.1), (256, 1, 1, 2, None)])
+def test_analytical_update_matches_oracle(cuda, n, s, m, b, samp):
+    """The reference's autograd=False update (Fourier projection + slice replay); at 256 x 256 with
+    0.1 A sampling the propagators run as row x column factors, at 2/256 A entry by entry."""
+    print(f"analytical n={n} s={s} m={m} sampling={samp}", flush=True)
+    case = _case_with_targets(N=n, S=s, M=m, B=b, seed=300 + n, descan=False, sampling=samp)
     dev = cuda
     props = _t(case["propagators"], dev) if s > 1 else None
     loss, og, pg = qp.analytical_gradients(_t(case["obj"], dev), _t(case["probe"], dev), _t(case["positions"], dev),
