@@ -49,8 +49,9 @@ struct Smem {
 // transformed; the footprint above half a CU's 160 KiB also keeps one workgroup per CU.
 struct SmemOne : Smem {
   cf dma[CH * N];  // the next chunk of W: rows [e][vector][j] or columns [e][j][column] (see dma_rows / dma_cols)
-  cf prow[N];      // row factor a[kr] of a separable propagator slice (see k_prop_factors256)
+  cf pfac[2 * N];  // row factor a[kr], then column factor b[kc] / N^2, of a separable propagator slice
 };
+static_assert(T == 2 * N, "one propagator factor per thread");
 static_assert(sizeof(SmemOne) > 80 * 1024 && sizeof(SmemOne) <= 160 * 1024, "one workgroup per CU");
 
 // LDS DMA (global_load_lds_dwordx4): every lane copies 16 bytes from its own global address to
@@ -533,10 +534,11 @@ __global__ __launch_bounds__(T) void k_forward256(Params p) {
       const cf* pr = p.prop + (size_t)s * N2;
       cf* xs = PG ? p.pspec + ((size_t)item * (p.S - 1) + s) * N2 : nullptr;
       cf pq[FAC ? 1 : 16];
-      cf pb;
       if constexpr (FAC) {
-        // separable: a[kr] from LDS (read after the pass's first barrier) times this lane's b[kc] / N^2
-        if (threadIdx.x < N) sm.prow[threadIdx.x] = p.pfac[(size_t)s * 2 * N + threadIdx.x];
+        // separable: a[kr] times b[kc] / N^2, both from LDS (one factor per thread, written before the
+        // pass's first barrier): no global load in the pass but its copies
+        const cf f = p.pfac[(size_t)s * 2 * N + threadIdx.x];
+        sm.pfac[threadIdx.x] = threadIdx.x < N ? f : cscale(f, inv_n2);
       }
       col_pass(
           Wb, Wb, sm,
@@ -549,7 +551,7 @@ __global__ __launch_bounds__(T) void k_forward256(Params p) {
             }
             if constexpr (FAC) {
 #pragma unroll
-              for (int e = 0; e < 16; ++e) v[e] = cmul(v[e], cmul(sm.prow[jj + 16 * e], pb));
+              for (int e = 0; e < 16; ++e) v[e] = cmul(v[e], cmul(sm.pfac[jj + 16 * e], sm.pfac[N + kc]));
             } else {
 #pragma unroll
               for (int e = 0; e < 16; ++e) v[e] = cscale(cmul(v[e], pq[e]), inv_n2);
@@ -558,9 +560,7 @@ __global__ __launch_bounds__(T) void k_forward256(Params p) {
             fft256<true, true>(v, sm);
           },
           [&](int kc) {
-            if constexpr (FAC) {
-              pb = cscale(p.pfac[(size_t)s * 2 * N + N + kc], inv_n2);
-            } else {
+            if constexpr (!FAC) {
 #pragma unroll
               for (int e = 0; e < 16; ++e) pq[e] = pr[(col_j() + 16 * e) * N + kc];
             }
@@ -686,11 +686,11 @@ __global__ __launch_bounds__(T) void k_adjoint256(Params p) {
       const cf* pr = p.prop + (size_t)(s - 1) * N2;
       cf* xs = PG ? p.pspec + ((size_t)item * (p.S - 1) + (s - 1)) * N2 : nullptr;
       cf q[FAC ? 1 : 16];
-      cf qb;
       cf xq[PG ? 16 : 1];
       if constexpr (FAC) {
-        // separable: conj(a[kr] b[kc] / N^2), a[kr] from LDS (read after the pass's first barrier)
-        if (threadIdx.x < N) sm.prow[threadIdx.x] = p.pfac[(size_t)(s - 1) * 2 * N + threadIdx.x];
+        // separable: conj(a[kr] b[kc] / N^2), both factors from LDS as in the forward
+        const cf f = p.pfac[(size_t)(s - 1) * 2 * N + threadIdx.x];
+        sm.pfac[threadIdx.x] = threadIdx.x < N ? f : cscale(f, inv_n2);
       }
       col_pass(
           Wb, s == p.s0 ? Xs : Wb, sm,
@@ -703,7 +703,7 @@ __global__ __launch_bounds__(T) void k_adjoint256(Params p) {
         }
         if constexpr (FAC) {
 #pragma unroll
-          for (int e = 0; e < 16; ++e) v[e] = cmulc(v[e], cmul(sm.prow[jj + 16 * e], qb));
+          for (int e = 0; e < 16; ++e) v[e] = cmulc(v[e], cmul(sm.pfac[jj + 16 * e], sm.pfac[N + kc]));
         } else {
 #pragma unroll
           for (int e = 0; e < 16; ++e) v[e] = cscale(cmulc(v[e], q[e]), inv_n2);
@@ -712,9 +712,7 @@ __global__ __launch_bounds__(T) void k_adjoint256(Params p) {
         fft256<true, true>(v, sm);
           },
           [&](int kc) {
-            if constexpr (FAC) {
-              qb = cscale(p.pfac[(size_t)(s - 1) * 2 * N + N + kc], inv_n2);
-            } else {
+            if constexpr (!FAC) {
 #pragma unroll
               for (int e = 0; e < 16; ++e) q[e] = pr[(col_j() + 16 * e) * N + kc];
             }
