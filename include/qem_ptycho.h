@@ -127,7 +127,7 @@ int qem_ptycho_forward_loss(const qem_ptycho_desc* desc, const void* obj, const 
  * counterpart in the reference.  The no-gradient forward only (desc->flags must be 0 or QEM_FLAG_FULL_PROPAGATORS: nothing is kept
  * for an adjoint); its workspace is a file-scope device array of QEM_STATIC_WORKSPACE_BYTES bytes that
  * each GPU holds from the first call on it until the process ends.  Requests whose workspace
- * (qem_ptycho_workspace_bytes with flags 0) exceeds it return QEM_ERR_WORKSPACE: at most 111 patterns
+ * (qem_ptycho_workspace_bytes with flags 0) exceeds it return QEM_ERR_WORKSPACE: at most 109 patterns
  * of 128 x 128 with 4 modes and 8 slices per call, 27 of 256 x 256 with 2 modes.  The call allocates nothing and
  * calls no allocating routine; the workspace's address is looked up (hipGetSymbolAddress) on the first
  * call on each device, for the device current at the call, which must be the device of `stream`.

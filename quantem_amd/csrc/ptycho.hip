@@ -36,6 +36,17 @@ template <int N>
 __global__ __launch_bounds__(Fft2<N>::T) void k_probe_spectrum(Params p) {
   __shared__ cf lds[Fft2<N>::LDS_TOTAL];
   using F = Fft2<N>;
+  const int ramp_blocks = (p.B * 2 * N + F::T - 1) / F::T;
+  if ((int)blockIdx.x >= p.M + ramp_blocks) {
+    // the blocks past the ramp table scale the propagators by 1 / N^2 (exactly: a power of two) for the
+    // multislice kernels, which then skip that multiply per element and slice
+    const int n = (p.S - 1) * N * N;
+    const float inv_n2 = 1.0f / (float)(N * N);
+    for (int idx = ((int)blockIdx.x - p.M - ramp_blocks) * F::T + (int)threadIdx.x; idx < n;
+         idx += ((int)gridDim.x - p.M - ramp_blocks) * F::T)
+      p.psc[idx] = cscale(p.prop[idx], inv_n2);
+    return;
+  }
   if ((int)blockIdx.x >= p.M) {
     // the blocks past the modes fill the shift-ramp table: ramps[b][0][k] = exp(-2 pi i f_k dr_b),
     // ramps[b][1][k] = exp(-2 pi i f_k dc_b), read by the forward prologue and the adjoint's last step
@@ -142,7 +153,6 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
   const int mode = p.fwd_mode;
   const bool far = mode != FWD_RECOMPUTE;
   const size_t HW = (size_t)p.H * p.W;
-  const float inv_n2 = 1.0f / (float)(N * N);
   // At N = 128 the object values of slice s + 1 are loaded inside the inverse transform of slice s's
   // propagation (after its last exchange, before its final register pass), which hides their L2
   // latency; at smaller N other resident waves hide it and the loads stay at the slice start.
@@ -175,7 +185,7 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
     if constexpr (QA > 0) {
       // the propagator values of the first QA registers are loaded before the wave stores; the last
       // slice (no propagation) reads the object's first N x N values instead, unused
-      const cf* prq = s + 1 < p.s1 ? p.prop + (size_t)s * N * N : p.obj;
+      const cf* prq = s + 1 < p.s1 ? p.psc + (size_t)s * N * N : p.obj;
 #pragma unroll
       for (int e = 0; e < QA; ++e) qa[e] = *f.p3_ptr(prq, e);
       __builtin_amdgcn_sched_barrier(0);
@@ -190,7 +200,7 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
     if (s + 1 < p.s1) {
       // propagation: the propagator values of each pass-3 register group are loaded before that
       // group's last transform stage, so that their L2 latency hides behind it
-      const cf* pr = p.prop + (size_t)s * N * N;
+      const cf* pr = p.psc + (size_t)s * N * N;  // propagator / N^2
       cf* xs = PG ? p.pspec + ((size_t)item * (p.S - 1) + s) * N * N : nullptr;
       cf q[F::RR3];
       f.fwd_groups(
@@ -206,7 +216,7 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_forward(Params p) {
             for (int i = 0; i < F::RR3; ++i) {
               const int e = h.value * F::RR3 + i;
               if constexpr (PG) st_stream(xs + f.p3_off(e), v[e]);
-              v[e] = cscale(cmul(v[e], e < QA ? qa[e < QA ? e : 0] : q[i]), inv_n2);
+              v[e] = cmul(v[e], e < QA ? qa[e < QA ? e : 0] : q[i]);
             }
           });
       const cf* On = O + HW;
@@ -412,7 +422,7 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
       }
       __builtin_amdgcn_sched_barrier(0);  // the object values are dead before the propagator loads
       // slice 0 has no propagation: its loads read the object's first N x N values, unused
-      const cf* prq = s > 0 ? p.prop + (size_t)(s - 1) * N * N : p.obj;
+      const cf* prq = s > 0 ? p.psc + (size_t)(s - 1) * N * N : p.obj;
 #pragma unroll
       for (int i = 0; i < F::RR3; ++i) qa[i] = *f.p3_ptr(prq, i);
       __builtin_amdgcn_sched_barrier(0);  // the first register group's loads are issued before the stores
@@ -454,7 +464,7 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
     if (s > 0) {
       // g_x <- ifft2(fft2(g_psi) conj(prop_{s-1})), the adjoint of the forward propagation, with the
       // propagator loads of each pass-3 register group issued before that group's last stage
-      const cf* __restrict__ pr = p.prop + (size_t)(s - 1) * N * N;
+      const cf* __restrict__ pr = p.psc + (size_t)(s - 1) * N * N;  // propagator / N^2
       cf* xs = PG ? p.pspec + ((size_t)item * (p.S - 1) + (s - 1)) * N * N : nullptr;
       cf q[F::RR3];
       cf xq[PG ? F::RR3 : 1];
@@ -476,7 +486,7 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
               const int e = h.value * F::RR3 + i;
               // dL/dP_{s-1} of this item: conj(fft2(x_{s-1})) fft2(g_psi_s) / N^2, over fft2(x_{s-1})
               if constexpr (PG) st_stream(f.p3_ptr(xs, e), cscale(cconjmul(xq[i], v[e]), inv_n2));
-              v[e] = cscale(cmulc(v[e], AHEAD && h.value == 0 ? qa[i] : q[i]), inv_n2);
+              v[e] = cmulc(v[e], AHEAD && h.value == 0 ? qa[i] : q[i]);
             }
           });
       if constexpr (PF > 0) {
@@ -1024,7 +1034,7 @@ void prof_mark(bool adjoint_kernel, int edge, hipStream_t st) {
 }
 
 struct Layout {
-  size_t psi, ckpt, scratch, X, gI, loss_part, pk, gsum, gpos, gdesc, ramps, pspec, pfac, pfac_ok, total;
+  size_t psi, ckpt, scratch, X, gI, loss_part, pk, gsum, gpos, gdesc, ramps, pspec, pfac, pfac_ok, psc, total;
   bool pfactors;  // separable-propagator factors checked and kept (N = 256, S > 1, no QEM_FLAG_FULL_PROPAGATORS)
   bool pgrad;  // propagator-gradient spectra kept (QEM_FLAG_GRAD_PROPAGATORS with stored waves)
   int nloss;
@@ -1081,6 +1091,8 @@ Layout layout_of(const qem_ptycho_desc& d) {
   if (L.pfactors) o += align_up((size_t)(S - 1) * 2 * d.roi * 8);
   L.pfac_ok = o;
   if (L.pfactors) o += align_up((size_t)(S - 1) * 8 * 4);  // one verdict per 32 rows of a slice
+  L.psc = o;
+  if (fixed_size(d) && d.roi <= 128 && S > 1) o += align_up((size_t)(S - 1) * N2 * 8);
   L.total = o;
   return L;
 }
@@ -1173,6 +1185,7 @@ qem::Params make_params(const qem_ptycho_desc& d, const Layout& L, void* ws) {
   p.pspec = L.pgrad ? (qem::cf*)(w + L.pspec) : nullptr;
   p.pfac = L.pfactors ? (qem::cf*)(w + L.pfac) : nullptr;
   p.pfac_ok = L.pfactors ? (int*)(w + L.pfac_ok) : nullptr;
+  p.psc = (fixed_size(d) && d.roi <= 128 && d.num_slices > 1) ? (qem::cf*)(w + L.psc) : nullptr;
   return p;
 }
 
@@ -1181,7 +1194,10 @@ int launch_forward(const qem::Params& p, const Layout& L, hipStream_t st) {
   using F = qem::Fft2<N>;
   // M probe-spectrum blocks, then the blocks of the (B, 2, N) shift-ramp table
   const int ramp_blocks = (p.B * 2 * N + F::T - 1) / F::T;
-  hipLaunchKernelGGL(qem::k_probe_spectrum<N>, dim3(p.M + ramp_blocks), dim3(F::T), 0, st, p);
+  // and the blocks that scale the propagators (at most 256, grid-stride)
+  const int n_prop = p.psc != nullptr ? (p.S - 1) * N * N : 0;
+  const int prop_blocks = std::min(256, (n_prop + F::T - 1) / F::T);
+  hipLaunchKernelGGL(qem::k_probe_spectrum<N>, dim3(p.M + ramp_blocks + prop_blocks), dim3(F::T), 0, st, p);
   if (int rc = check_launch("probe_spectrum")) return rc;
   prof_mark(false, 0, st);
   // every incident wave kept: the store-every-slice instantiation (no branch around the stores)

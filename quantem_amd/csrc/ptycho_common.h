@@ -61,6 +61,9 @@ struct Params {
   // 32-row part of the slice ((S-1, 8) ints, 1 = yes); nullptr = apply every propagator entry as given
   cf* pfac;
   int* pfac_ok;
+  // N <= 128: the propagators times 1 / N^2 ((S-1, N, N), filled by the probe-spectrum launch; a power of
+  // two, so every product with them equals the product with the propagator, scaled)
+  cf* psc;
 };
 
 // Layout signature of Params as compiled in the current translation unit: its size and every field's
@@ -74,7 +77,7 @@ struct Params {
   QF(targets) QF(mask) QF(descan) QF(loss_out) QF(intens_out) QF(dloss) QF(grad_obj) QF(grad_probe)           \
   QF(grad_pos) QF(grad_descan) QF(psi) QF(scratch) QF(X) QF(gI) QF(loss_part) QF(pk) QF(gsum) QF(ramps)          \
   QF(gpos_part) QF(gdesc_part) QF(s0) QF(s1) QF(wslots) QF(ck) QF(nck) QF(fwd_mode) QF(ckpt) QF(pspec)  \
-  QF(pfac) QF(pfac_ok)
+  QF(pfac) QF(pfac_ok) QF(psc)
 // internal linkage: each translation unit evaluates its own copy (an inline function would be merged)
 static constexpr unsigned long long params_signature_local() {
   unsigned long long h = sizeof(Params);

@@ -74,12 +74,12 @@ def test_static_forward_rejects_before_touching_the_device():
     """Host-side refusals (no pointer is dereferenced, no device call is made): gradient flags, and a
     batch whose workspace exceeds the static one."""
     lib = nat.load()
-    d = nat.QemDesc(num_slices=8, num_modes=4, batch=112, roi=128, obj_rows=384, obj_cols=384, num_gpts=4096,
+    d = nat.QemDesc(num_slices=8, num_modes=4, batch=110, roi=128, obj_rows=384, obj_cols=384, num_gpts=4096,
                     loss_type=0, mean_intensity=1.0, flags=0, segment_slices=0)
     args = [None] * 10
     assert lib.qem_ptycho_forward_loss_static(ctypes.byref(d), *args) == nat.QEM_ERR_WORKSPACE
     assert "static workspace" in nat.last_error()
-    d.batch = 111
+    d.batch = 109
     assert lib.qem_ptycho_forward_loss_static(ctypes.byref(d), *args) == nat.QEM_ERR_NULL
     d.flags = nat.QEM_FLAG_STORE_WAVES
     assert lib.qem_ptycho_forward_loss_static(ctypes.byref(d), *args) == nat.QEM_ERR_ARG
