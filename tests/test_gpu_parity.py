@@ -364,13 +364,15 @@ def test_headline_size_batch_is_the_sum_of_its_parts(cuda):
     assert rel(full[1][:2].cpu().numpy(), sub) < FWD_TOL
 
 
-@pytest.mark.parametrize("n,s,m,b,descan", [(32, 3, 2, 4, True), (64, 4, 2, 3, False), (128, 3, 2, 3, False),
-                                            (128, 8, 4, 2, False), (256, 3, 2, 2, True)])
-def test_propagator_gradient_matches_oracle(cuda, n, s, m, b, descan):
+@pytest.mark.parametrize("n,s,m,b,descan,samp", [(32, 3, 2, 4, True, None), (64, 4, 2, 3, False, None),
+                                                 (128, 3, 2, 3, False, None), (128, 8, 4, 2, False, None),
+                                                 (256, 3, 2, 2, True, None), (256, 3, 2, 2, True, 0.1)])
+def test_propagator_gradient_matches_oracle(cuda, n, s, m, b, descan, samp):
     """dL/dpropagators (the gradient a learned probe tilt chains through) against the FP64 oracle; the
-    other gradients are unchanged by the extra output."""
-    print(f"propagator gradient n={n} s={s} m={m} b={b} descan={descan}", flush=True)
-    case = _case_with_targets(N=n, S=s, M=m, B=b, seed=900 + n + s, descan=descan)
+    other gradients are unchanged by the extra output.  At 256 x 256 with 0.1 A sampling the propagators
+    factor, so the gradient comes from the kernels that apply them as row x column factors."""
+    print(f"propagator gradient n={n} s={s} m={m} b={b} descan={descan} sampling={samp}", flush=True)
+    case = _case_with_targets(N=n, S=s, M=m, B=b, seed=900 + n + s, descan=descan, sampling=samp)
     dev = cuda
     obj = _t(case["obj"], dev).requires_grad_(True)
     probe = _t(case["probe"], dev).requires_grad_(True)

@@ -1,7 +1,8 @@
 """The allocation-free forward (qem_ptycho_forward_loss_static) against the workspace forward
 (qem_ptycho_forward_loss): same kernels, so loss and intensities are bitwise equal; two calls in a row
-on the shared static workspace, the second with a smaller input, at 128 x 128, at 256 x 256 then 64 x 64,
-and at the generic shapes 96 x 80 then 30 x 42."""
+on the shared static workspace, the second with a smaller input, at 128 x 128, at 256 x 256 then 64 x 64
+(the 256 x 256 propagators once entry by entry, once as row x column factors at 0.1 A sampling), and at
+the generic shapes 96 x 80 then 30 x 42."""
 
 import ctypes
 
@@ -19,8 +20,8 @@ def _ptr(t):
     return None if t is None else t.data_ptr()
 
 
-def _inputs(n, s, m, b, seed, dev):
-    case = make_case(N=n, S=s, M=m, B=b, seed=seed)
+def _inputs(n, s, m, b, seed, dev, sampling=None):
+    case = make_case(N=n, S=s, M=m, B=b, seed=seed, sampling=sampling)
     I = po.forward(case["obj"], case["probe"], case["positions"], case["propagators"])["I"]
     case = attach_targets(case, I)
     t = {k: torch.from_numpy(np.ascontiguousarray(case[k])).to(dev)
@@ -58,11 +59,12 @@ def _forward_static(d, t, dev):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("first,second", [((128, 8, 4, 40), (128, 3, 2, 9)), ((256, 3, 2, 12), (64, 2, 1, 5)),
+                                          ((256, 3, 2, 12, 0.1), (64, 2, 1, 5)),
                                           (((96, 80), 3, 2, 20), ((30, 42), 2, 1, 6))])
 def test_static_forward_matches_workspace_forward(cuda, first, second):
-    for k, (n, s, m, b) in enumerate((first, second)):
-        print(f"static forward call {k}: shape={n} s={s} m={m} b={b}", flush=True)
-        d, t = _inputs(n, s, m, b, 40 + k + (n if isinstance(n, int) else sum(n)), cuda)
+    for k, (n, s, m, b, *samp) in enumerate((first, second)):
+        print(f"static forward call {k}: shape={n} s={s} m={m} b={b} sampling={samp}", flush=True)
+        d, t = _inputs(n, s, m, b, 40 + k + (n if isinstance(n, int) else sum(n)), cuda, *samp)
         ref_loss, ref_I = _forward_workspace(d, t, cuda)
         loss, I = _forward_static(d, t, cuda)
         torch.cuda.synchronize()
