@@ -4,10 +4,10 @@
 // (pattern, mode) item and keeps its wave in a per-item buffer W, in the "row-transformed" domain
 // (r, kc): every row already Fourier transformed.  A slice then costs two passes over W:
 //
-//   row pass     for 16 rows at a time: inverse row DFT -> natural-order wave (r, c) -> the
+//   row pass     for 32 rows at a time: inverse row DFT -> natural-order wave (r, c) -> the
 //                multislice elementwise work (store the incident wave, transmission, descan, the
 //                adjoint's gradient patch) -> forward row DFT -> back to W
-//   column pass  for 16 columns at a time: column DFT -> full 2-D spectrum (kr, kc) -> propagator
+//   column pass  for 32 columns at a time: column DFT -> full 2-D spectrum (kr, kc) -> propagator
 //                (or far field / probe-gradient epilogue) -> inverse column DFT -> back to W
 //
 // A length-256 DFT is 16 x 16: a radix-16 DFT in registers (16 lanes per row or column, 16 values
