@@ -197,7 +197,7 @@ int qem_profile_end(float* fwd_ms, int32_t* n_fwd, float* adj_ms, int32_t* n_adj
 /* Thread-local description of the last error (empty string if none). */
 const char* qem_last_error(void);
 
-/* ABI version (major * 100 + minor): 103. */
+/* ABI version (major * 100 + minor): 104. */
 int32_t qem_abi_version(void);
 
 #ifdef __cplusplus

@@ -1467,7 +1467,7 @@ int forward_loss_checked(const qem_ptycho_desc& d, const void* obj, const void* 
 
 extern "C" {
 
-int32_t qem_abi_version(void) { return 103; }
+int32_t qem_abi_version(void) { return 104; }
 
 int qem_profile_begin(int32_t max_launches) {
   qem::clear_error();

@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
     lib = nat.load()
     for name in declared_functions():
         assert hasattr(lib, name), name
-    assert lib.qem_abi_version() == 103
+    assert lib.qem_abi_version() == 104
 
 
 def _desc(**kw):
