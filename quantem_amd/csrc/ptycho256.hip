@@ -48,18 +48,12 @@ struct Smem {
 // the exchange buffer, one chunk of W (64 KiB) that the LDS DMA copies in while the previous chunk is
 // transformed; the footprint above half a CU's 160 KiB also keeps one workgroup per CU.
 struct SmemOne : Smem {
-  cf dma[CH * N];  // the next chunk of W: rows [e][vector][j] or columns [e][j][column] (see dma_rows / dma_cols)
+  cf dma[CH * N];  // the next chunk of W: rows per wave, columns [e][j][column] (see dma_rows / dma_cols)
   cf pfac[2 * N];  // row factor a[kr], then column factor b[kc] / N^2, of a separable propagator slice
 };
 static_assert(T == 2 * N, "one propagator factor per thread");
 static_assert(sizeof(SmemOne) > 80 * 1024 && sizeof(SmemOne) <= 160 * 1024, "one workgroup per CU");
 
-// LDS DMA (global_load_lds_dwordx4): every lane copies 16 bytes from its own global address to
-// consecutive 16-byte slots of a 1 KiB LDS block.  A chunk of W (CH vectors x 256 values, 64 KiB) is
-// 64 such blocks, 8 per wave.  The copy bypasses the registers, so the next chunk is fetched while the
-// current one is transformed.  The waits are explicit: a pass waits with vmcnt(16) for its own copy,
-// allowing only the previous chunk's 16 row / column stores (the last vector-memory instructions of
-// the wave) to stay in flight; loads and stores share the one in-order vmcnt counter.
 // Orders this wave's LDS accesses: the lanes of one wave issue LDS instructions together and the LDS
 // serves one wave's instructions in order, so a read after this sees every write of the wave before
 // it; only the compiler has to be kept from moving accesses across (as rocPRIM's wave_barrier).
@@ -69,6 +63,12 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// LDS DMA (global_load_lds_dwordx4): every lane copies 16 bytes from its own global address to
+// consecutive 16-byte slots of a 1 KiB LDS block.  A chunk of W (CH vectors x 256 values, 64 KiB) is
+// 64 such blocks, 8 per wave.  The copy bypasses the registers, so the next chunk is fetched while the
+// current one is transformed.  The waits are explicit: a pass waits with vmcnt(16) for its own copy,
+// allowing only the previous chunk's 16 row / column stores (the last vector-memory instructions of
+// the wave) to stay in flight; loads and stores share the one in-order vmcnt counter.
 typedef __attribute__((address_space(3))) void* lds_void_ptr;
 
 // One copy instruction: 16 bytes per lane from base + lane_off to the LDS block at lds (its 64 lanes'
@@ -353,7 +353,8 @@ __global__ __launch_bounds__(T) void k_spectrum256(Params p) {
 // [32 part, 32 part + 32) of slice s: a[kr] = P[kr][0] and b[kc] = P[0][kc] / P[0][0] (quotient in
 // double, rounded once), a[kr] b[kc] formed in float as the kernels do, and the part marked separable
 // when none of its entries differs from it by more than 1e-6 of the largest magnitude among them (a
-// non-finite entry, or P[0][0] = 0, marks it not separable).  1024 threads: thread t reads column
+// non-finite entry or factor, or P[0][0] = 0, marks it not separable, so NaNs reach the results as
+// they would entry by entry).  1024 threads: thread t reads column
 // t % 256 of every fourth row of the part from t / 256.
 constexpr int kFacT = 1024, kFacParts = 8, kFacRows = N / kFacParts;
 __global__ __launch_bounds__(kFacT) void k_prop_factors256(Params p) {
@@ -385,8 +386,9 @@ __global__ __launch_bounds__(kFacT) void k_prop_factors256(Params p) {
     const cf q = cmul(a_s[g + G * i], b_c);
     const double ex = (double)q.x - (double)pv[i].x, ey = (double)q.y - (double)pv[i].y;
     const double e = ex * ex + ey * ey, m = (double)pv[i].x * pv[i].x + (double)pv[i].y * pv[i].y;
-    if (!(e <= dev)) dev = e;  // a NaN sticks
-    if (!(m <= mag)) mag = m;
+    // a NaN counts as an infinite deviation (or magnitude), and an infinity stays the maximum
+    if (!(e <= dev)) dev = e == e ? e : INFINITY;
+    if (!(m <= mag)) mag = m == m ? m : INFINITY;
   }
   for (int o = 32; o > 0; o >>= 1) {
     const double od = __shfl_down(dev, o, 64), om = __shfl_down(mag, o, 64);
@@ -403,8 +405,8 @@ __global__ __launch_bounds__(kFacT) void k_prop_factors256(Params p) {
       if (!(red[0][w] <= dev)) dev = red[0][w];
       if (!(red[1][w] <= mag)) mag = red[1][w];
     }
-    // squared magnitudes: deviation <= 1e-6 * largest magnitude of the part
-    p.pfac_ok[s * kFacParts + part] = (d > 0.0 && dev <= 1e-12 * mag) ? 1 : 0;
+    // squared magnitudes: deviation <= 1e-6 * largest magnitude of the part, both finite
+    p.pfac_ok[s * kFacParts + part] = (d > 0.0 && mag < INFINITY && dev <= 1e-12 * mag) ? 1 : 0;
   }
 }
 

@@ -178,6 +178,21 @@ def test_separable_propagators_256(cuda):
         assert np.array_equal(got[k], full[k]), k
 
 
+@pytest.mark.parametrize("bad", [float("nan"), float("inf")])
+def test_nonfinite_propagator_entry_takes_the_entry_path_256(cuda, bad):
+    """A separable 256 x 256 propagator with one non-finite entry away from its first row and column
+    (where the factors would not see it) is not applied through factors: the intensities equal, NaNs
+    included, those with every entry applied as given."""
+    print(f"non-finite propagator entry {bad}", flush=True)
+    case = _case_with_targets(N=256, S=3, M=1, B=2, seed=44, sampling=0.1)
+    case["propagators"][1, 5, 7] = bad
+    with torch.no_grad():
+        got = _run(case, cuda, grads=False)
+        full = _run(case, cuda, grads=False, full_propagators=True)
+    assert not np.all(np.isfinite(got["I"]))
+    assert np.array_equal(got["I"], full["I"], equal_nan=True)
+
+
 def test_depth_20_slices_256px_against_oracle_and_finite_differences(cuda):
     """The deep-adjoint configuration (20 slices of 512x512, 1 A, 256x256 patterns, 2 modes) on two
     scan positions, with every incident wave stored and with 5-slice recomputation segments:
