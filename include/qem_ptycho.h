@@ -65,14 +65,17 @@ enum {
   QEM_FLAG_GRAD_PROPAGATORS = 1u << 4, /* with QEM_FLAG_STORE_WAVES and segment_slices 0: keep what
                                          qem_ptycho_propagator_gradient needs (one more (S-1)-slice
                                          spectrum per pattern and mode in the workspace) */
-  QEM_FLAG_FULL_PROPAGATORS = 1u << 5  /* 256 x 256 patterns: apply every propagator entry as given.
+  QEM_FLAG_FULL_PROPAGATORS = 1u << 5  /* 256 x 256 patterns (every pass) and 128 x 128 patterns (the
+                                         adjoint without descan or propagator gradient): apply every
+                                         propagator entry as given.
                                          By default a propagator slice that is the product of a row
                                          and a column factor (the reference's Fresnel propagators,
                                          tilted or not: exp(-i pi lambda dz (kr^2 + kc^2)) times the
                                          tilt's per-axis terms) is applied as that product, the
                                          factors taken from its first column and first row, when for
                                          every slice the product reproduces each entry within 1e-6 of
-                                         the largest magnitude among its 32 rows (checked on the GPU by
+                                         the largest magnitude among its 32 rows (16 at 128 x 128;
+                                         checked on the GPU by
                                          qem_ptycho_forward_loss, kept in the workspace for the
                                          adjoint); otherwise every slice is applied entry by entry. */
 };

@@ -32,18 +32,102 @@ namespace qem {
 
 // ---------------------------------------------------------------------------------------------
 // Probe spectra: pk[m] = FFT2(probe[m]).
+// Separable propagators at N = 128.  The reference's propagator is exp(-i pi lambda dz (kr^2 + kc^2))
+// times, for a tilt, one phase term per axis: a row factor times a column factor.  Per slice the
+// workspace keeps a[r] = P[r][0] / N^2 and b[c] = P[0][c] / P[0][0] (kFacParts128 verdicts of 16 rows
+// each say whether a[r] b[c] reproduces every entry within 1e-6 of the largest magnitude of the rows);
+// the adjoint then reads both factors from LDS instead of N^2 entries from global memory per slice.
+// The row factors are stored in the order of the adjoint's pass-3 registers: slot j * RR3 + i holds
+// row j + RR1 * RR2 * i, so that register group h of lane-group q reads slots (q * H3 + h) * RR3 + i.
+constexpr int kFacParts128 = 8;
+
+template <int N>
+__device__ __forceinline__ void prop_factors128(const Params& p, int blk) {
+  using F = Fft2<N>;
+  static_assert(F::T == 4 * N && N / kFacParts128 == 16, "four row groups of 16 rows per part");
+  __shared__ double red[2][F::T / 64];
+  const int s = blk / kFacParts128, part = blk - s * kFacParts128, t = threadIdx.x;
+  const int c = t & (N - 1), g = t / N;
+  const cf* P = p.prop + (size_t)s * N * N;
+  cf* fa = p.pfac + (size_t)s * 2 * N;
+  const cf p00 = P[0], p0c = P[c];
+  const double d = (double)p00.x * p00.x + (double)p00.y * p00.y;
+  const cf b_c = d > 0.0 ? mk((float)(((double)p0c.x * p00.x + (double)p0c.y * p00.y) / d),
+                              (float)(((double)p0c.y * p00.x - (double)p0c.x * p00.y) / d))
+                         : mk(0.0f, 0.0f);
+  if (part == 0 && t < 2 * N) {
+    if (t < N) {
+      const int row = t / F::RR3 + F::RR1 * F::RR2 * (t % F::RR3);
+      fa[t] = cscale(P[(size_t)row * N], 1.0f / (float)(N * N));
+    } else {
+      fa[t] = b_c;
+    }
+  }
+  double dev = 0.0, mag = 0.0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int r = part * 16 + g + 4 * i;
+    const cf pv = P[(size_t)r * N + c];
+    const cf q = cmul(P[(size_t)r * N], b_c);
+    const double ex = (double)q.x - (double)pv.x, ey = (double)q.y - (double)pv.y;
+    const double e = ex * ex + ey * ey, m = (double)pv.x * pv.x + (double)pv.y * pv.y;
+    // a NaN counts as an infinite deviation (or magnitude), and an infinity stays the maximum
+    if (!(e <= dev)) dev = e == e ? e : INFINITY;
+    if (!(m <= mag)) mag = m == m ? m : INFINITY;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double od = __shfl_down(dev, o, 64), om = __shfl_down(mag, o, 64);
+    if (!(od <= dev)) dev = od;
+    if (!(om <= mag)) mag = om;
+  }
+  if ((t & 63) == 0) {
+    red[0][t >> 6] = dev;
+    red[1][t >> 6] = mag;
+  }
+  __syncthreads();
+  if (t == 0) {
+    for (int w = 1; w < F::T / 64; ++w) {
+      if (!(red[0][w] <= dev)) dev = red[0][w];
+      if (!(red[1][w] <= mag)) mag = red[1][w];
+    }
+    // squared magnitudes: deviation <= 1e-6 * largest magnitude of the part, both finite
+    p.pfac_ok[s * kFacParts128 + part] = (d > 0.0 && mag < INFINITY && dev <= 1e-12 * mag) ? 1 : 0;
+  }
+}
+
+// Whether every propagator slice of the launch is separable (the verdicts of prop_factors128).
+__device__ __forceinline__ bool all_separable128(const Params& p) {
+  if (p.pfac == nullptr) return false;
+  const int n = (p.S - 1) * kFacParts128;
+  bool all = true;
+  for (int i0 = 0; i0 < n; i0 += 64) {
+    const int i = i0 + (int)(threadIdx.x & 63);
+    all = all && __ballot(i < n && p.pfac_ok[i] == 0) == 0ull;
+  }
+  return all;
+}
+
 template <int N>
 __global__ __launch_bounds__(Fft2<N>::T) void k_probe_spectrum(Params p) {
   __shared__ cf lds[Fft2<N>::LDS_TOTAL];
   using F = Fft2<N>;
   const int ramp_blocks = (p.B * 2 * N + F::T - 1) / F::T;
-  if ((int)blockIdx.x >= p.M + ramp_blocks) {
-    // the blocks past the ramp table scale the propagators by 1 / N^2 (exactly: a power of two) for the
-    // multislice kernels, which then skip that multiply per element and slice
+  const int fac_blocks = (N == 128 && p.pfac != nullptr) ? (p.S - 1) * kFacParts128 : 0;
+  if constexpr (N == 128) {
+    const int blk = (int)blockIdx.x - p.M - ramp_blocks;
+    if (blk >= 0 && blk < fac_blocks) {
+      // the blocks after the ramp table form and check the propagators' separable factors
+      prop_factors128<N>(p, blk);
+      return;
+    }
+  }
+  if ((int)blockIdx.x >= p.M + ramp_blocks + fac_blocks) {
+    // the last blocks scale the propagators by 1 / N^2 (exactly: a power of two) for the multislice
+    // kernels, which then skip that multiply per element and slice
     const int n = (p.S - 1) * N * N;
     const float inv_n2 = 1.0f / (float)(N * N);
-    for (int idx = ((int)blockIdx.x - p.M - ramp_blocks) * F::T + (int)threadIdx.x; idx < n;
-         idx += ((int)gridDim.x - p.M - ramp_blocks) * F::T)
+    const int first = p.M + ramp_blocks + fac_blocks;
+    for (int idx = ((int)blockIdx.x - first) * F::T + (int)threadIdx.x; idx < n; idx += ((int)gridDim.x - first) * F::T)
       p.psc[idx] = cscale(p.prop[idx], inv_n2);
     return;
   }
@@ -319,10 +403,12 @@ __global__ __launch_bounds__(256) void k_loss_finalize(Params p, int nparts) {
 // At s = 0: G = fft2(g_psi_0) and the probe-gradient spectrum conj(ramp_b) G is stored over X;
 // the position gradient reduces Im(conj(G) P_m ramp_b) against the frequencies.
 // PG: also the propagator-gradient contributions (over p.pspec, see Params).
-template <int N, bool DESCAN, bool PG = false>
-__global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
-  __shared__ cf lds[Fft2<N>::LDS_TOTAL];
-  __shared__ double red[Fft2<N>::T / 64];
+// One (pattern, mode) item of the adjoint.  FAC (N = 128, no descan, no propagator gradient): the
+// propagations apply the separable factors of prop_factors128, read from fab (4N slots: the factors of
+// two propagations, double-buffered by slice parity, row factors then column factors).
+template <int N, bool DESCAN, bool PG, bool FAC>
+__device__ __forceinline__ void adjoint_item(const Params& p, cf* __restrict__ lds, double* __restrict__ red,
+                                             cf* __restrict__ fab) {
   using F = Fft2<N>;
   constexpr int PTS = F::PTS;
   F f(threadIdx.x);
@@ -336,6 +422,14 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
   const float dl = (p.flags & QEM_FLAG_ANALYTIC) ? 1.0f : p.dloss[0];
   const float inv_n = 1.0f / (float)N;
   const float inv_n2 = 1.0f / (float)(N * N);
+  // With FAC the first 2N threads (four whole waves) copy the factors of propagator j into LDS: each
+  // loads its value one slice ahead (fload, beside the next slice's wave prefetch) and writes it at
+  // the start of the slice that uses it.  The load is branch-free (the other threads load a copy).
+  cf fv = mk(0.0f, 0.0f);
+  const bool fwriter = __builtin_amdgcn_readfirstlane((int)threadIdx.x) < 2 * N;
+  auto fload = [&](int j) {
+    if constexpr (FAC) fv = p.pfac[(size_t)(j > 0 ? j : 0) * 2 * N + (threadIdx.x & (2 * N - 1))];
+  };
 
   cf v[PTS];
   cf* Xs = p.X + (size_t)item * N * N;
@@ -422,9 +516,11 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
       }
       __builtin_amdgcn_sched_barrier(0);  // the object values are dead before the propagator loads
       // slice 0 has no propagation: its loads read the object's first N x N values, unused
-      const cf* prq = s > 0 ? p.psc + (size_t)(s - 1) * N * N : p.obj;
+      if constexpr (!FAC) {
+        const cf* prq = s > 0 ? p.psc + (size_t)(s - 1) * N * N : p.obj;
 #pragma unroll
-      for (int i = 0; i < F::RR3; ++i) qa[i] = *f.p3_ptr(prq, i);
+        for (int i = 0; i < F::RR3; ++i) qa[i] = *f.p3_ptr(prq, i);
+      }
       __builtin_amdgcn_sched_barrier(0);  // the first register group's loads are issued before the stores
 #pragma unroll
       for (int i = 0; i < PTS; i += 2) st_stream2(f.wv_ptr(ps, i), pre[i], pre[i + 1]);
@@ -459,7 +555,12 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
 #pragma unroll
     for (int i = 0; i < PTS; ++i) ob[i] = obj_at(p.obj + (size_t)(p.s1 - 1) * HW, oo.at(i));
   }
+  fload(p.s1 - 2);
   for (int s = p.s1 - 1; s >= p.s0; --s) {
+    // propagator s - 1's factors into LDS (this buffer's last reads, two slices up, are separated from
+    // here by that slice's exchange barriers; the reads below follow the forward transform's barriers)
+    if constexpr (FAC)
+      if (fwriter) fab[((s - 1) & 1) * 2 * N + threadIdx.x] = fv;
     slice_step(s, s < p.s1 - 1);
     if (s > 0) {
       // g_x <- ifft2(fft2(g_psi) conj(prop_{s-1})), the adjoint of the forward propagation, with the
@@ -467,11 +568,15 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
       const cf* __restrict__ pr = p.psc + (size_t)(s - 1) * N * N;  // propagator / N^2
       cf* xs = PG ? p.pspec + ((size_t)item * (p.S - 1) + (s - 1)) * N * N : nullptr;
       cf q[F::RR3];
+      cf fb = mk(0.0f, 0.0f);  // column factor b[kc] (FAC)
+      const cf* fa = FAC ? fab + ((s - 1) & 1) * 2 * N + f.q * F::H3 * F::RR3 : fab;  // this lane group's row factors
       cf xq[PG ? F::RR3 : 1];
       f.fwd_groups(
           v, lds,
           [&](auto h) {
-            if constexpr (!AHEAD || h.value > 0) {
+            if constexpr (FAC) {
+              if constexpr (h.value == 0) fb = fab[((s - 1) & 1) * 2 * N + N + f.kc];
+            } else if constexpr (!AHEAD || h.value > 0) {
 #pragma unroll
               for (int i = 0; i < F::RR3; ++i) q[i] = *f.p3_ptr(pr, h.value * F::RR3 + i);
             }
@@ -486,7 +591,10 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
               const int e = h.value * F::RR3 + i;
               // dL/dP_{s-1} of this item: conj(fft2(x_{s-1})) fft2(g_psi_s) / N^2, over fft2(x_{s-1})
               if constexpr (PG) st_stream(f.p3_ptr(xs, e), cscale(cconjmul(xq[i], v[e]), inv_n2));
-              v[e] = cmulc(v[e], AHEAD && h.value == 0 ? qa[i] : q[i]);
+              if constexpr (FAC)
+                v[e] = cmulc(cmulc(v[e], fa[h.value * F::RR3 + i]), fb);
+              else
+                v[e] = cmulc(v[e], AHEAD && h.value == 0 ? qa[i] : q[i]);
             }
           });
       if constexpr (PF > 0) {
@@ -498,6 +606,7 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
 #pragma unroll
         for (int i = 0; i < PF; i += 2) ld_stream2(f.wv_ptr(pn, i), pre[i], pre[i + 1]);
       }
+      fload(s - 2);
       const cf* On = p.obj + (size_t)(s - 1) * HW;
       f.inv_hooked(v, lds, [&] {
         if constexpr (AHEAD) {
@@ -561,6 +670,24 @@ __global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
       p.gpos_part[2 * item + 1] = (float)(2.0 * M_PI * tc / (double)(N * N));
     }
   }
+}
+
+// The adjoint kernel.  At N = 128 without descan or propagator gradient it takes the separable-factor
+// path when every propagator slice of the launch passed the factor check (a uniform branch: one
+// launch, and the workgroup's LDS holds the factor tables as well).
+template <int N, bool DESCAN, bool PG = false>
+__global__ __launch_bounds__(Fft2<N>::T) void k_adjoint(Params p) {
+  constexpr bool FACTORS = N == 128 && !DESCAN && !PG;
+  __shared__ cf lds[Fft2<N>::LDS_TOTAL];
+  __shared__ double red[Fft2<N>::T / 64];
+  __shared__ cf fab[FACTORS ? 4 * N : 1];
+  if constexpr (FACTORS) {
+    if (all_separable128(p)) {
+      adjoint_item<N, DESCAN, PG, true>(p, lds, red, fab);
+      return;
+    }
+  }
+  adjoint_item<N, DESCAN, PG, false>(p, lds, red, fab);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1035,7 +1162,7 @@ void prof_mark(bool adjoint_kernel, int edge, hipStream_t st) {
 
 struct Layout {
   size_t psi, ckpt, scratch, X, gI, loss_part, pk, gsum, gpos, gdesc, ramps, pspec, pfac, pfac_ok, psc, total;
-  bool pfactors;  // separable-propagator factors checked and kept (N = 256, S > 1, no QEM_FLAG_FULL_PROPAGATORS)
+  bool pfactors;  // separable-propagator factors checked and kept (N = 128, 256, S > 1, no QEM_FLAG_FULL_PROPAGATORS)
   bool pgrad;  // propagator-gradient spectra kept (QEM_FLAG_GRAD_PROPAGATORS with stored waves)
   int nloss;
   int wslots;  // wave slots per item in psi
@@ -1086,13 +1213,13 @@ Layout layout_of(const qem_ptycho_desc& d) {
   L.pgrad = store && (d.flags & QEM_FLAG_GRAD_PROPAGATORS) && S > 1;
   L.pspec = o;
   if (L.pgrad) o += align_up(BM * (size_t)(S - 1) * N2 * 8);
-  L.pfactors = fixed_size(d) && d.roi == 256 && S > 1 && !(d.flags & QEM_FLAG_FULL_PROPAGATORS);
+  L.psc = o;
+  if (fixed_size(d) && d.roi <= 128 && S > 1) o += align_up((size_t)(S - 1) * N2 * 8);
+  L.pfactors = fixed_size(d) && (d.roi == 256 || d.roi == 128) && S > 1 && !(d.flags & QEM_FLAG_FULL_PROPAGATORS);
   L.pfac = o;
   if (L.pfactors) o += align_up((size_t)(S - 1) * 2 * d.roi * 8);
   L.pfac_ok = o;
-  if (L.pfactors) o += align_up((size_t)(S - 1) * 8 * 4);  // one verdict per 32 rows of a slice
-  L.psc = o;
-  if (fixed_size(d) && d.roi <= 128 && S > 1) o += align_up((size_t)(S - 1) * N2 * 8);
+  if (L.pfactors) o += align_up((size_t)(S - 1) * 8 * 4);  // eight verdicts per slice
   L.total = o;
   return L;
 }
@@ -1197,7 +1324,10 @@ int launch_forward(const qem::Params& p, const Layout& L, hipStream_t st) {
   // and the blocks that scale the propagators (at most 256, grid-stride)
   const int n_prop = p.psc != nullptr ? (p.S - 1) * N * N : 0;
   const int prop_blocks = std::min(256, (n_prop + F::T - 1) / F::T);
-  hipLaunchKernelGGL(qem::k_probe_spectrum<N>, dim3(p.M + ramp_blocks + prop_blocks), dim3(F::T), 0, st, p);
+  // and, at N = 128, the blocks that form and check the propagators' separable factors
+  const int fac_blocks = (N == 128 && p.pfac != nullptr) ? (p.S - 1) * qem::kFacParts128 : 0;
+  hipLaunchKernelGGL(qem::k_probe_spectrum<N>, dim3(p.M + ramp_blocks + fac_blocks + prop_blocks), dim3(F::T), 0,
+                     st, p);
   if (int rc = check_launch("probe_spectrum")) return rc;
   prof_mark(false, 0, st);
   // every incident wave kept: the store-every-slice instantiation (no branch around the stores)
@@ -1337,6 +1467,15 @@ int launch_reductions(const qem::Params& p, hipStream_t st) {
   return QEM_OK;
 }
 
+// The adjoint without descan or propagator gradient (at N = 128 it picks the separable-factor path
+// itself, by the factor verdicts).
+template <int N>
+int launch_adjoint_plain(const qem::Params& q, hipStream_t st) {
+  using F = qem::Fft2<N>;
+  hipLaunchKernelGGL((qem::k_adjoint<N, false>), dim3(q.B * q.M), dim3(F::T), 0, st, q);
+  return check_launch("adjoint");
+}
+
 template <int N>
 int launch_adjoint(const qem::Params& p, hipStream_t st) {
   using F = qem::Fft2<N>;
@@ -1347,15 +1486,17 @@ int launch_adjoint(const qem::Params& p, hipStream_t st) {
       hipLaunchKernelGGL((qem::k_forward<N, false, true>), dim3(q.B * q.M), dim3(F::T), 0, st, q);
       if (int r = check_launch("forward (recompute)")) return r;
     }
-    if (q.pspec != nullptr && q.descan != nullptr)
-      hipLaunchKernelGGL((qem::k_adjoint<N, true, true>), dim3(q.B * q.M), dim3(F::T), 0, st, q);
-    else if (q.pspec != nullptr)
-      hipLaunchKernelGGL((qem::k_adjoint<N, false, true>), dim3(q.B * q.M), dim3(F::T), 0, st, q);
-    else if (q.descan != nullptr)
-      hipLaunchKernelGGL((qem::k_adjoint<N, true>), dim3(q.B * q.M), dim3(F::T), 0, st, q);
-    else
-      hipLaunchKernelGGL((qem::k_adjoint<N, false>), dim3(q.B * q.M), dim3(F::T), 0, st, q);
-    if (int r = check_launch("adjoint")) return r;
+    if (q.pspec == nullptr && q.descan == nullptr) {
+      if (int r = launch_adjoint_plain<N>(q, st)) return r;
+    } else {
+      if (q.pspec != nullptr && q.descan != nullptr)
+        hipLaunchKernelGGL((qem::k_adjoint<N, true, true>), dim3(q.B * q.M), dim3(F::T), 0, st, q);
+      else if (q.pspec != nullptr)
+        hipLaunchKernelGGL((qem::k_adjoint<N, false, true>), dim3(q.B * q.M), dim3(F::T), 0, st, q);
+      else
+        hipLaunchKernelGGL((qem::k_adjoint<N, true>), dim3(q.B * q.M), dim3(F::T), 0, st, q);
+      if (int r = check_launch("adjoint")) return r;
+    }
     return q.ck ? launch_obj_reduce<N>(q, st) : QEM_OK;
   });
   prof_mark(true, 1, st);
@@ -1393,8 +1534,7 @@ int launch_projection_update(const qem::Params& p, const Layout& L, float* overl
     hipLaunchKernelGGL(qem::k_patch_energy<N>, dim3(p.S * p.B), dim3(256), 0, st, p, energy);
     if (int rc = check_launch("patch_energy")) return rc;
   }
-  hipLaunchKernelGGL((qem::k_adjoint<N, false>), dim3(p.B * p.M), dim3(F::T), 0, st, p);
-  if (int rc = check_launch("adjoint")) return rc;
+  if (int rc = launch_adjoint_plain<N>(p, st)) return rc;
   if (int rc = launch_obj_reduce<N>(p, st)) return rc;
   hipLaunchKernelGGL(qem::k_probe_reduce<N>, dim3(p.M * N * N / 64), dim3(256), 0, st, p);
   if (int rc = check_launch("probe_reduce")) return rc;

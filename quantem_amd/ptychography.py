@@ -247,10 +247,11 @@ def multislice_loss(
         segment_slices: 0 keeps every slice's incident wave for the backward pass; k in [2, S) keeps
             only every k-th one and recomputes the others in the backward pass, one k-slice segment
             at a time (less memory, one extra propagation per slice).
-        full_propagators: 256 x 256 patterns apply a propagator slice that factors into a row and a
-            column term (the reference's Fresnel propagators, tilted or not) as that product, after
-            checking on the GPU that it reproduces every entry within 1e-6 of the largest; True
-            applies every entry as given (``QEM_FLAG_FULL_PROPAGATORS``).
+        full_propagators: 256 x 256 patterns (and the 128 x 128 adjoint without descan or
+            propagator gradient) apply a propagator slice that factors into a row and a column term
+            (the reference's Fresnel propagators, tilted or not) as that product, after checking on
+            the GPU that it reproduces every entry within 1e-6 of the largest; True applies every
+            entry as given (``QEM_FLAG_FULL_PROPAGATORS``).
 
     Returns:
         loss (0-d float32 tensor), and intensities if requested.

@@ -178,6 +178,48 @@ def test_separable_propagators_256(cuda):
         assert np.array_equal(got[k], full[k]), k
 
 
+def test_separable_propagators_128(cuda):
+    """The 128 x 128 adjoint (no descan, no propagator gradient) applies a separable propagator slice
+    through its row and column factors by default.  The forward applies every entry in both runs, so
+    the intensities are bitwise those of QEM_FLAG_FULL_PROPAGATORS; the gradients differ in the last
+    bits (the factor path ran) and stay within 2e-5 of the entry-by-entry ones, and both paths meet the
+    oracle bar.  One slice multiplied by a random phase sends the adjoint down the entry-by-entry path:
+    gradients bitwise those of QEM_FLAG_FULL_PROPAGATORS."""
+    print("separable propagators n=128 s=4 m=2 b=3", flush=True)
+    case = _case_with_targets(N=128, S=4, M=2, B=3, seed=51, sampling=0.1)
+    fac = _run(case, cuda, desc_grad=False)
+    full = _run(case, cuda, desc_grad=False, full_propagators=True)
+    ref = _oracle(case)
+    keys = ("obj", "probe", "positions")
+    print("  factors vs entries: " + ", ".join(f"{k} {rel(fac[k], full[k]):.2e}" for k in keys), flush=True)
+    for name, got in (("factors", fac), ("entries", full)):
+        print(f"  {name} vs oracle: " + ", ".join(f"{k} {rel(got[k], ref[k]):.2e}" for k in ("I",) + keys), flush=True)
+    assert np.array_equal(fac["I"], full["I"])
+    assert not np.array_equal(fac["obj"], full["obj"])
+    for k in keys:
+        assert rel(fac[k], full[k]) < 2e-5, k
+    for got in (fac, full):
+        assert rel(got["I"], ref["I"]) < FWD_TOL
+        for k in keys:
+            assert rel(got[k], ref[k]) < GRAD_TOL, k
+
+    print("one slice not separable", flush=True)
+    mixed = make_case(N=128, S=4, M=2, B=3, seed=52, sampling=0.1)
+    rng = np.random.default_rng(53)
+    mixed["propagators"][2] *= np.exp(0.3j * rng.uniform(-1.0, 1.0, (128, 128))).astype(np.complex64)
+    I = po.forward(mixed["obj"], mixed["probe"], mixed["positions"], mixed["propagators"])["I"]
+    mixed = attach_targets(mixed, I)
+    got = _run(mixed, cuda, desc_grad=False)
+    full = _run(mixed, cuda, desc_grad=False, full_propagators=True)
+    ref = _oracle(mixed)
+    errs = {k: rel(got[k], ref[k]) for k in ("I",) + keys}
+    print("  " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
+    assert errs["I"] < FWD_TOL
+    for k in keys:
+        assert errs[k] < GRAD_TOL, k
+        assert np.array_equal(got[k], full[k]), k
+
+
 @pytest.mark.parametrize("bad", [float("nan"), float("inf")])
 def test_nonfinite_propagator_entry_takes_the_entry_path_256(cuda, bad):
     """A separable 256 x 256 propagator with one non-finite entry away from its first row and column
