@@ -989,39 +989,41 @@ __global__ __launch_bounds__(N) void k_obj_reduce_rows(Params p) {
   for (int x = t; x < p.W; x += N) out[x] = acc_row[x];
 }
 
-// gsum[m][k] = sum_b spectra[b][m][k] in a fixed order: 64 frequencies per block, 4 lanes per
-// frequency each summing every 4th pattern, combined in lane order through LDS.
+// gsum[m][k] = sum_b spectra[b][m][k] in a fixed order: 128 frequencies per block, two adjacent ones per
+// lane (one 16-byte load per pattern), 4 lanes per frequency pair each summing every 4th pattern,
+// combined in lane order through LDS.
 template <int N>
 __global__ __launch_bounds__(256) void k_probe_reduce(Params p) {
-  __shared__ cf part[4][64];
+  __shared__ float4 part[4][64];
   const int kk = threadIdx.x & 63, g = threadIdx.x >> 6;
-  const int idx = blockIdx.x * 64 + kk;  // over M*N*N (multiple of 64)
+  const int idx = blockIdx.x * 128 + 2 * kk;  // over M*N*N (multiple of 128)
   const int m = idx / (N * N);
   const int k = idx - m * N * N;
-  cf acc = mk(0.0f, 0.0f);
-  const cf* src = p.X + (size_t)m * N * N + k;
-  const size_t stride = (size_t)p.M * N * N;
+  float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const float4* src = reinterpret_cast<const float4*>(p.X + (size_t)m * N * N + k);
+  const size_t stride = (size_t)p.M * N * N / 2;  // one pattern, in float4
 #pragma unroll 8
   for (int b = g; b < p.B; b += 4) {
-    const cf t = src[(size_t)b * stride];
+    const float4 t = src[(size_t)b * stride];
     acc.x += t.x;
     acc.y += t.y;
+    acc.z += t.z;
+    acc.w += t.w;
   }
   part[g][kk] = acc;
   __syncthreads();
   if (g == 0) {
-    cf t = part[0][kk];
+    float4 t = part[0][kk];
     for (int i = 1; i < 4; ++i) {
       t.x += part[i][kk].x;
       t.y += part[i][kk].y;
+      t.z += part[i][kk].z;
+      t.w += part[i][kk].w;
     }
-    p.gsum[(size_t)idx] = t;
+    reinterpret_cast<float4*>(p.gsum)[(size_t)idx / 2] = t;
   }
 }
 
-// grad_prop[s][k] = sum over the items (b, m) of their contributions in p.pspec, in a fixed order:
-// 64 frequencies of one slice per block, 4 lanes per frequency each summing every 4th item, combined in
-// lane order through LDS.
 template <int N>
 __global__ __launch_bounds__(256) void k_prop_reduce(Params p, cf* grad_prop) {
   __shared__ cf part[4][64];
@@ -1431,7 +1433,7 @@ int launch_adjoint256(const qem::Params& p, hipStream_t st) {
   if (rc) return rc;
   if (p.ck == 0)
     if (int r = launch_obj_reduce<256>(p, st)) return r;
-  hipLaunchKernelGGL(qem::k_probe_reduce<256>, dim3(p.M * 256 * 256 / 64), dim3(256), 0, st, p);
+  hipLaunchKernelGGL(qem::k_probe_reduce<256>, dim3(p.M * 256 * 256 / 128), dim3(256), 0, st, p);
   if (int rc = check_launch("probe_reduce")) return rc;
   if (int rc = qem::launch_probe_grad256(p, st)) return rc;
   return launch_reductions(p, st);
@@ -1503,7 +1505,7 @@ int launch_adjoint(const qem::Params& p, hipStream_t st) {
   if (rc) return rc;
   if (p.ck == 0)
     if (int r = launch_obj_reduce<N>(p, st)) return r;
-  hipLaunchKernelGGL(qem::k_probe_reduce<N>, dim3(p.M * N * N / 64), dim3(256), 0, st, p);
+  hipLaunchKernelGGL(qem::k_probe_reduce<N>, dim3(p.M * N * N / 128), dim3(256), 0, st, p);
   if (int rc = check_launch("probe_reduce")) return rc;
   hipLaunchKernelGGL(qem::k_probe_grad<N>, dim3(p.M), dim3(F::T), 0, st, p);
   if (int rc = check_launch("probe_grad")) return rc;
@@ -1536,7 +1538,7 @@ int launch_projection_update(const qem::Params& p, const Layout& L, float* overl
   }
   if (int rc = launch_adjoint_plain<N>(p, st)) return rc;
   if (int rc = launch_obj_reduce<N>(p, st)) return rc;
-  hipLaunchKernelGGL(qem::k_probe_reduce<N>, dim3(p.M * N * N / 64), dim3(256), 0, st, p);
+  hipLaunchKernelGGL(qem::k_probe_reduce<N>, dim3(p.M * N * N / 128), dim3(256), 0, st, p);
   if (int rc = check_launch("probe_reduce")) return rc;
   hipLaunchKernelGGL(qem::k_probe_grad<N>, dim3(p.M), dim3(F::T), 0, st, p);
   return check_launch("probe_grad");
@@ -1557,7 +1559,7 @@ int launch_projection_update256(const qem::Params& p, const Layout& L, float* ov
   }
   if (int rc = qem::launch_chain256_adjoint(p, st)) return rc;
   if (int rc = launch_obj_reduce<256>(p, st)) return rc;
-  hipLaunchKernelGGL(qem::k_probe_reduce<256>, dim3(p.M * 256 * 256 / 64), dim3(256), 0, st, p);
+  hipLaunchKernelGGL(qem::k_probe_reduce<256>, dim3(p.M * 256 * 256 / 128), dim3(256), 0, st, p);
   if (int rc = check_launch("probe_reduce")) return rc;
   return qem::launch_probe_grad256(p, st);
 }
